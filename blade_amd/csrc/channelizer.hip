@@ -1,0 +1,441 @@
+// FFT channelizer:  out[a, f*C + k, s, p] = fftshift(DFT_C(in[a, f, s*C:(s+1)*C, p]))[k]
+// (oracle tests/modules/channelizer/advanced.py:49-65; forward, unnormalised,
+// e^{-j 2pi k n / C}).  Written from the Stockham autosort formulation of the
+// mixed-radix FFT (each pass reads stride C/R, twiddles by W_{Ns R}^{(j mod Ns) r},
+// butterflies, writes to (j/Ns)*Ns*R + j mod Ns + r*Ns), so the output lands
+// in natural order without a bit-reversal pass.  fftshift is an index remap on
+// the store.  Twiddles come from a per-C table built in FP64 and rounded
+// once to FP32.
+#include "common.hh"
+
+#include <map>
+#include <mutex>
+#include <vector>
+
+namespace bl {
+
+// ---------------------------------------------------------------------------
+// Twiddle tables  W_C^m = exp(-j 2 pi m / C),  m in [0, C), one per (device, C).
+// ---------------------------------------------------------------------------
+
+__global__ void twiddle_kernel(float2* tw, int C) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < C) {
+        double s, c;
+        sincospi(-2.0 * (double)m / (double)C, &s, &c);
+        tw[m] = make_float2((float)c, (float)s);
+    }
+}
+
+static std::mutex g_tw_mutex;
+static std::map<std::pair<int, int>, float2*> g_tw;
+
+static int get_twiddles(int C, hipStream_t s, const float2** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail("channelizer: hipGetDevice failed");
+    std::lock_guard<std::mutex> lock(g_tw_mutex);
+    auto it = g_tw.find({dev, C});
+    if (it != g_tw.end()) {
+        *out = it->second;
+        return BLADE_SUCCESS;
+    }
+    float2* p = nullptr;
+    hipError_t e = hipMalloc(&p, sizeof(float2) * C);
+    if (e != hipSuccess) {
+        set_error(std::string("channelizer: twiddle allocation: ") + hipGetErrorString(e));
+        return BLADE_HIP_ERROR;
+    }
+    hipLaunchKernelGGL(twiddle_kernel, dim3((C + 255) / 256), dim3(256), 0, s, p, C);
+    int rc = check_launch("channelizer twiddles");
+    if (rc) return rc;
+    g_tw[{dev, C}] = p;
+    *out = p;
+    return BLADE_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// Small DFT in registers.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float2 mul_neg_j(float2 a) { return make_float2(a.y, -a.x); }
+
+template <int R>
+__device__ __forceinline__ void dft_reg(float2 (&v)[R], const float2* __restrict__ tw, int twstep) {
+    if constexpr (R == 1) {
+        return;
+    } else if constexpr (R == 2) {
+        const float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 4) {
+        const float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        const float2 s13 = cadd(v[1], v[3]), d13 = mul_neg_j(csub(v[1], v[3]));
+        v[0] = cadd(s02, s13);
+        v[2] = csub(s02, s13);
+        v[1] = cadd(d02, d13);
+        v[3] = csub(d02, d13);
+    } else if constexpr (R == 8) {
+        float2 e[4] = {v[0], v[2], v[4], v[6]};
+        float2 o[4] = {v[1], v[3], v[5], v[7]};
+        dft_reg<4>(e, tw, 0);
+        dft_reg<4>(o, tw, 0);
+        const float c = 0.70710678118654752440f;
+        const float2 o1 = make_float2(c * (o[1].x + o[1].y), c * (o[1].y - o[1].x));   // * e^{-j pi/4}
+        const float2 o2 = mul_neg_j(o[2]);                                            // * e^{-j pi/2}
+        const float2 o3 = make_float2(c * (o[3].y - o[3].x), -c * (o[3].x + o[3].y)); // * e^{-j 3pi/4}
+        v[0] = cadd(e[0], o[0]); v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);   v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);   v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);   v[7] = csub(e[3], o3);
+    } else {
+        // Direct DFT of size R with W_R^q = tw[q * twstep].
+        float2 r[R];
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            float2 acc = v[0];
+#pragma unroll
+            for (int n = 1; n < R; n++) {
+                const float2 w = tw[((n * k) % R) * twstep];
+                acc.x += v[n].x * w.x - v[n].y * w.y;
+                acc.y += v[n].x * w.y + v[n].y * w.x;
+            }
+            r[k] = acc;
+        }
+#pragma unroll
+        for (int k = 0; k < R; k++) v[k] = r[k];
+    }
+}
+
+// Constant-twiddle DFT for the register kernel (W computed at compile time
+// from the table argument in a small struct).
+template <int C>
+struct SmallTw {
+    float2 w[C];
+};
+
+template <int C>
+__device__ __forceinline__ void dft_small(float2 (&v)[C], const SmallTw<C>& tw) {
+    if constexpr (C == 2 || C == 4 || C == 8) {
+        dft_reg<C>(v, nullptr, 0);
+    } else {
+        float2 r[C];
+#pragma unroll
+        for (int k = 0; k < C; k++) {
+            float2 acc = v[0];
+#pragma unroll
+            for (int n = 1; n < C; n++) {
+                const float2 w = tw.w[(n * k) % C];
+                acc.x += v[n].x * w.x - v[n].y * w.y;
+                acc.y += v[n].x * w.y + v[n].y * w.x;
+            }
+            r[k] = acc;
+        }
+#pragma unroll
+        for (int k = 0; k < C; k++) v[k] = r[k];
+    }
+}
+
+// One lane per C-sample spectrum (all P pols), C <= 16.  Reads C*P contiguous
+// complex samples, writes one sample per output row (coalesced across lanes).
+template <int C, int P>
+__global__ __launch_bounds__(256) void chan_small_kernel(const float2* __restrict__ in,
+                                                         float2* __restrict__ out,
+                                                         uint64_t n_spec, uint64_t S,
+                                                         SmallTw<C> tw) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t sp = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; sp < n_spec; sp += stride) {
+        const uint64_t af = sp / S, s = sp - af * S;
+        float2 v[P][C];
+        const float2* src = in + sp * C * P;
+#pragma unroll
+        for (int n = 0; n < C; n++) {
+            if constexpr (P == 2) {
+                const float4 q = reinterpret_cast<const float4*>(src)[n];
+                v[0][n] = make_float2(q.x, q.y);
+                v[1][n] = make_float2(q.z, q.w);
+            } else {
+                v[0][n] = src[n];
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < P; p++) dft_small<C>(v[p], tw);
+#pragma unroll
+        for (int k = 0; k < C; k++) {
+            const int ko = (k + C / 2) % C;
+            const uint64_t o = ((af * C + ko) * S + s) * P;
+            if constexpr (P == 2) {
+                reinterpret_cast<float4*>(out + o)[0] =
+                    make_float4(v[0][k].x, v[0][k].y, v[1][k].x, v[1][k].y);
+            } else {
+                out[o] = v[0][k];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LDS Stockham kernel: one workgroup per spectrum (all P pols), C up to 8192
+// (P = 2) or 16384 (P = 1).
+// ---------------------------------------------------------------------------
+
+struct Radices {
+    int r[16];
+    int n;
+};
+
+constexpr int kMaxLdsComplex = 16384;   // 128 KiB of LDS
+
+template <int R>
+constexpr int groups_max() {
+    return (kMaxLdsComplex / R + 1023) / 1024;
+}
+
+template <int R>
+__device__ __forceinline__ void stockham_pass(float2* __restrict__ data, int C, int PC, int Ns,
+                                              const float2* __restrict__ tw) {
+    constexpr int G = groups_max<R>();
+    const int nthr = blockDim.x;
+    const int per_pol = C / R;
+    const int total = PC / R;
+    const int twstride = C / (Ns * R);
+    float2 v[G][R];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        const int gi = threadIdx.x + g * nthr;
+        if (gi < total) {
+            const int p = gi / per_pol, j = gi - p * per_pol;
+            const float2* base = data + p * C;
+            const int jm = j % Ns;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                float2 x = base[j + r * per_pol];
+                if (r > 0 && jm > 0) {
+                    const float2 w = tw[jm * r * twstride];
+                    x = cmul(x, w);
+                }
+                v[g][r] = x;
+            }
+            dft_reg<R>(v[g], tw, C / R);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        const int gi = threadIdx.x + g * nthr;
+        if (gi < total) {
+            const int p = gi / per_pol, j = gi - p * per_pol;
+            float2* base = data + p * C;
+            const int jm = j % Ns;
+            const int d = (j - jm) * R + jm;
+#pragma unroll
+            for (int r = 0; r < R; r++) base[d + r * Ns] = v[g][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int P>
+__global__ __launch_bounds__(1024) void chan_lds_kernel(const float2* __restrict__ in,
+                                                        float2* __restrict__ out, int C,
+                                                        uint64_t S, uint64_t n_spec,
+                                                        const float2* __restrict__ tw,
+                                                        Radices rad) {
+    extern __shared__ float2 data[];
+    const int PC = P * C;
+    for (uint64_t sp = blockIdx.x; sp < n_spec; sp += gridDim.x) {
+        const uint64_t af = sp / S, s = sp - af * S;
+        const float2* src = in + sp * (uint64_t)C * P;
+        for (int n = threadIdx.x; n < C; n += blockDim.x) {
+            if constexpr (P == 2) {
+                const float4 q = reinterpret_cast<const float4*>(src)[n];
+                data[n] = make_float2(q.x, q.y);
+                data[C + n] = make_float2(q.z, q.w);
+            } else {
+                data[n] = src[n];
+            }
+        }
+        __syncthreads();
+        int Ns = 1;
+        for (int i = 0; i < rad.n; i++) {
+            const int R = rad.r[i];
+            switch (R) {
+                case 2: stockham_pass<2>(data, C, PC, Ns, tw); break;
+                case 3: stockham_pass<3>(data, C, PC, Ns, tw); break;
+                case 4: stockham_pass<4>(data, C, PC, Ns, tw); break;
+                case 5: stockham_pass<5>(data, C, PC, Ns, tw); break;
+                case 7: stockham_pass<7>(data, C, PC, Ns, tw); break;
+                default: stockham_pass<8>(data, C, PC, Ns, tw); break;
+            }
+            Ns *= R;
+        }
+        const int half = C / 2;
+        for (int k = threadIdx.x; k < C; k += blockDim.x) {
+            int ko = k + half;
+            if (ko >= C) ko -= C;
+            const uint64_t o = ((af * C + ko) * S + s) * P;
+            if constexpr (P == 2) {
+                reinterpret_cast<float4*>(out + o)[0] =
+                    make_float4(data[k].x, data[k].y, data[C + k].x, data[C + k].y);
+            } else {
+                out[o] = data[k];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Direct O(C^2) DFT for rates with prime factors above 7.
+template <int P>
+__global__ __launch_bounds__(256) void chan_direct_kernel(const float2* __restrict__ in,
+                                                          float2* __restrict__ out, int C,
+                                                          uint64_t S, uint64_t n_spec,
+                                                          const float2* __restrict__ tw) {
+    extern __shared__ float2 data[];
+    for (uint64_t sp = blockIdx.x; sp < n_spec; sp += gridDim.x) {
+        const uint64_t af = sp / S, s = sp - af * S;
+        const float2* src = in + sp * (uint64_t)C * P;
+        for (int i = threadIdx.x; i < C * P; i += blockDim.x) {
+            const int n = i / P, p = i - n * P;
+            data[p * C + n] = src[i];
+        }
+        __syncthreads();
+        const int half = C / 2;
+        for (int i = threadIdx.x; i < C * P; i += blockDim.x) {
+            const int k = i / P, p = i - k * P;
+            float2 acc = make_float2(0.f, 0.f);
+            int m = 0;
+            for (int n = 0; n < C; n++) {
+                const float2 x = data[p * C + n], w = tw[m];
+                acc.x += x.x * w.x - x.y * w.y;
+                acc.y += x.x * w.y + x.y * w.x;
+                m += k;
+                if (m >= C) m -= C;
+            }
+            int ko = k + half;
+            if (ko >= C) ko -= C;
+            out[((af * C + ko) * S + s) * P + p] = acc;
+        }
+        __syncthreads();
+    }
+}
+
+static bool factorize(int C, Radices* rad) {
+    rad->n = 0;
+    int c = C;
+    // Prefer radix 8, then 4, then one radix 2; then 3, 5, 7.
+    while (c % 8 == 0 && rad->n < 16) { rad->r[rad->n++] = 8; c /= 8; }
+    while (c % 4 == 0 && rad->n < 16) { rad->r[rad->n++] = 4; c /= 4; }
+    while (c % 2 == 0 && rad->n < 16) { rad->r[rad->n++] = 2; c /= 2; }
+    while (c % 3 == 0 && rad->n < 16) { rad->r[rad->n++] = 3; c /= 3; }
+    while (c % 5 == 0 && rad->n < 16) { rad->r[rad->n++] = 5; c /= 5; }
+    while (c % 7 == 0 && rad->n < 16) { rad->r[rad->n++] = 7; c /= 7; }
+    return c == 1;
+}
+
+template <int C, int P>
+static void launch_small(const float2* in, float2* out, uint64_t n_spec, uint64_t S, hipStream_t s) {
+    SmallTw<C> tw;
+    for (int m = 0; m < C; m++) {
+        const double a = -2.0 * M_PI * (double)m / (double)C;
+        tw.w[m] = make_float2((float)cos(a), (float)sin(a));
+    }
+    uint64_t blocks = ceil_div(n_spec, 256);
+    if (blocks > 256 * 32) blocks = 256 * 32;
+    hipLaunchKernelGGL((chan_small_kernel<C, P>), dim3((unsigned)blocks), dim3(256), 0, s, in, out,
+                       n_spec, S, tw);
+}
+
+template <int P>
+static int dispatch_small(int C, const float2* in, float2* out, uint64_t n_spec, uint64_t S,
+                          hipStream_t s) {
+    switch (C) {
+        case 2: launch_small<2, P>(in, out, n_spec, S, s); return 1;
+        case 4: launch_small<4, P>(in, out, n_spec, S, s); return 1;
+        case 6: launch_small<6, P>(in, out, n_spec, S, s); return 1;
+        case 8: launch_small<8, P>(in, out, n_spec, S, s); return 1;
+        case 10: launch_small<10, P>(in, out, n_spec, S, s); return 1;
+        case 12: launch_small<12, P>(in, out, n_spec, S, s); return 1;
+        case 14: launch_small<14, P>(in, out, n_spec, S, s); return 1;
+        case 16: launch_small<16, P>(in, out, n_spec, S, s); return 1;
+    }
+    return 0;
+}
+
+}  // namespace bl
+
+using namespace bl;
+
+extern "C" int blade_channelizer(const void* input, void* output, uint64_t A, uint64_t F,
+                                 uint64_t T, uint64_t P, uint64_t rate, blade_stream_t stream) {
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    // Validation mirrors src/modules/channelizer/base.cc:17-42 except that any
+    // rate dividing T is accepted (the oracle's general semantics).
+    if (rate == 0 || T % rate) return fail("channelizer: the number of time samples should be divisible by the channelizer rate");
+    if (rate % 2 && rate != 1) return fail("channelizer: the channelizer rate should be divisible by 2");
+    if (P != 1 && P != 2) return fail("channelizer: number of polarizations of the input should be one or two");
+    const uint64_t n = A * F * T * P;
+    if (n == 0) return BLADE_SUCCESS;
+    if (!input || !output) return fail("channelizer: null pointer");
+    if ((uintptr_t)input % 16 || (uintptr_t)output % 16) return fail("channelizer: misaligned buffer");
+    const int C = (int)rate;
+    const uint64_t S = T / rate;
+    const uint64_t n_spec = A * F * S;
+    if (C == 1) {
+        if (input == output) return BLADE_SUCCESS;
+        hipError_t e = hipMemcpyAsync(output, input, n * 8, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) {
+            set_error(std::string("channelizer copy: ") + hipGetErrorString(e));
+            return BLADE_HIP_ERROR;
+        }
+        return BLADE_SUCCESS;
+    }
+    if (input == output && S != 1)
+        return fail("channelizer: in-place operation needs rate == number of time samples");
+    const float2* in = static_cast<const float2*>(input);
+    float2* out = static_cast<float2*>(output);
+    if (C <= 16 && input != output) {
+        const int ok = (P == 2) ? dispatch_small<2>(C, in, out, n_spec, S, s)
+                                : dispatch_small<1>(C, in, out, n_spec, S, s);
+        if (ok) return check_launch("channelizer");
+    }
+    if ((uint64_t)C * P > (uint64_t)kMaxLdsComplex)
+        return fail("channelizer: rate * polarizations above 16384 is not supported");
+    const float2* tw = nullptr;
+    int rc = get_twiddles(C, s, &tw);
+    if (rc) return rc;
+    Radices rad;
+    const size_t lds = sizeof(float2) * (size_t)C * P;
+    uint64_t blocks = n_spec;
+    if (blocks > 256 * 64) blocks = 256 * 64;
+    const void* fn = nullptr;
+    int threads = 256;
+    if (factorize(C, &rad)) {
+        threads = 64;
+        while (threads < 1024 && threads * 8 < C * (int)P) threads *= 2;
+        fn = (P == 2) ? reinterpret_cast<const void*>(&chan_lds_kernel<2>)
+                      : reinterpret_cast<const void*>(&chan_lds_kernel<1>);
+    } else {
+        fn = (P == 2) ? reinterpret_cast<const void*>(&chan_direct_kernel<2>)
+                      : reinterpret_cast<const void*>(&chan_direct_kernel<1>);
+    }
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) {
+            set_error(std::string("channelizer: LDS attribute: ") + hipGetErrorString(e));
+            return BLADE_HIP_ERROR;
+        }
+    }
+    if (fn == reinterpret_cast<const void*>(&chan_lds_kernel<2>))
+        hipLaunchKernelGGL((chan_lds_kernel<2>), dim3((unsigned)blocks), dim3(threads), lds, s,
+                           in, out, C, S, n_spec, tw, rad);
+    else if (fn == reinterpret_cast<const void*>(&chan_lds_kernel<1>))
+        hipLaunchKernelGGL((chan_lds_kernel<1>), dim3((unsigned)blocks), dim3(threads), lds, s,
+                           in, out, C, S, n_spec, tw, rad);
+    else if (P == 2)
+        hipLaunchKernelGGL((chan_direct_kernel<2>), dim3((unsigned)blocks), dim3(256), lds, s,
+                           in, out, C, S, n_spec, tw);
+    else
+        hipLaunchKernelGGL((chan_direct_kernel<1>), dim3((unsigned)blocks), dim3(256), lds, s,
+                           in, out, C, S, n_spec, tw);
+    return check_launch("channelizer");
+}

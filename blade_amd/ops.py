@@ -1,0 +1,189 @@
+"""Tensor-level entry points over the C-ABI, on torch device memory.
+
+PyTorch only provides device buffers and the current HIP stream; every
+computation is a call into ``libblade_amd.so``.  Conventions:
+
+* CF32 -> ``torch.complex64``; CF64 -> ``torch.complex128``;
+* CI8 -> ``torch.int8`` with a trailing axis of 2 (re, im);
+* CF16 -> ``torch.float16`` with a trailing axis of 2.
+"""
+
+from __future__ import annotations
+
+import ctypes as ct
+
+import torch
+
+from . import _lib
+from ._lib import call
+
+__all__ = ["type_of", "stream_handle", "cast", "beamformer", "beamform_chain", "channelizer",
+           "polarizer", "detector", "gather", "permutation", "phasor_ata"]
+
+
+def stream_handle(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ct.c_void_p(s.cuda_stream)
+
+
+def _ptr(t: torch.Tensor):
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return ct.c_void_p(t.data_ptr())
+
+
+def type_of(t: torch.Tensor) -> str:
+    """blade type name of a tensor under the conventions above."""
+    if t.dtype == torch.complex64:
+        return "cf32"
+    if t.dtype == torch.complex128:
+        return "cf64"
+    if t.dtype == torch.int8:
+        return "ci8" if t.dim() and t.shape[-1] == 2 else "i8"
+    if t.dtype == torch.float16:
+        return "cf16" if t.dim() and t.shape[-1] == 2 else "f16"
+    return {torch.float32: "f32", torch.float64: "f64", torch.int32: "i32"}[t.dtype]
+
+
+def _elements(t: torch.Tensor, type_name: str) -> int:
+    if type_name in ("ci8", "cf16"):
+        return t.numel() // 2
+    return t.numel()
+
+
+def _shape4(t: torch.Tensor, type_name: str):
+    shape = tuple(t.shape[:-1]) if type_name in ("ci8", "cf16") else tuple(t.shape)
+    if len(shape) != 4:
+        raise ValueError(f"expected a rank-4 [A,F,T,P] tensor, got {shape}")
+    return shape
+
+
+_TORCH_OF = {"i8": torch.int8, "f16": torch.float16, "f32": torch.float32, "f64": torch.float64,
+             "i32": torch.int32, "ci8": torch.int8, "cf16": torch.float16,
+             "cf32": torch.complex64, "cf64": torch.complex128}
+
+
+def empty(shape, type_name: str, device="cuda"):
+    shape = tuple(shape)
+    if type_name in ("ci8", "cf16"):
+        shape = shape + (2,)
+    return torch.empty(shape, dtype=_TORCH_OF[type_name], device=device)
+
+
+def cast(x: torch.Tensor, out_type: str, out: torch.Tensor | None = None, in_type: str | None = None,
+         stream=None) -> torch.Tensor:
+    """Component-wise cast (Modules::Cast)."""
+    it = in_type or type_of(x)
+    n = _elements(x, it)
+    base_shape = tuple(x.shape[:-1]) if it in ("ci8", "cf16") else tuple(x.shape)
+    if out is None:
+        out = empty(base_shape, out_type, x.device)
+    call("blade_cast", _lib.TYPE_CODES[it], _lib.TYPE_CODES[out_type], _ptr(x), _ptr(out), n,
+         stream_handle(stream))
+    return out
+
+
+def beamformer(x: torch.Tensor, phasors: torch.Tensor, incoherent=False, incoherent_sqrt=False,
+               precision="f32", out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+    """ATA beamformer (Modules::Beamformer::ATA); x CF32 or CI8 [A,F,T,P], phasors CF32 [B,A,F,1,P]."""
+    return beamform_chain(x, phasors, incoherent, incoherent_sqrt, 1, False, precision, out, stream)
+
+
+def beamform_chain(x: torch.Tensor, phasors: torch.Tensor, incoherent=False, incoherent_sqrt=False,
+                   rate=1, polarize=False, precision="f32", out: torch.Tensor | None = None,
+                   stream=None) -> torch.Tensor:
+    """Fused beamform -> channelize(rate 1/2/4) -> [polarize]."""
+    it = type_of(x)
+    A, F, T, P = _shape4(x, it)
+    B = phasors.shape[0]
+    if tuple(phasors.shape) != (B, A, F, 1, P) or phasors.dtype != torch.complex64:
+        raise ValueError(f"phasors must be CF32 [B,{A},{F},1,{P}], got {tuple(phasors.shape)}")
+    nb = B + (1 if incoherent else 0)
+    if out is None:
+        out = torch.empty((nb, F * rate, T // rate, P), dtype=torch.complex64, device=x.device)
+    prec = {"f32": _lib.PREC_F32, "f16x2": _lib.PREC_F16X2}[precision]
+    if rate == 1 and not polarize:
+        call("blade_beamformer_ata", _lib.TYPE_CODES[it], prec, _ptr(x), _ptr(phasors), _ptr(out),
+             A, F, T, P, B, int(incoherent), int(incoherent_sqrt), stream_handle(stream))
+    else:
+        call("blade_beamform_chain", _lib.TYPE_CODES[it], prec, _ptr(x), _ptr(phasors), _ptr(out),
+             A, F, T, P, B, int(incoherent), int(incoherent_sqrt), rate, int(polarize),
+             stream_handle(stream))
+    return out
+
+
+def channelizer(x: torch.Tensor, rate: int, out: torch.Tensor | None = None, stream=None):
+    """FFT channelizer (Modules::Channelizer), CF32 [A,F,T,P] -> [A,F*C,T/C,P]."""
+    A, F, T, P = _shape4(x, "cf32")
+    if out is None:
+        out = torch.empty((A, F * rate, T // rate, P), dtype=torch.complex64, device=x.device)
+    call("blade_channelizer", _ptr(x), _ptr(out), A, F, T, P, rate, stream_handle(stream))
+    return out
+
+
+def polarizer(x: torch.Tensor, out: torch.Tensor | None = None, stream=None):
+    """XY -> LR (Modules::Polarizer), CF32 or CF16."""
+    t = type_of(x)
+    if out is None:
+        out = torch.empty_like(x)
+    call("blade_polarizer", _lib.TYPE_CODES[t], _ptr(x), _ptr(out), _elements(x, t),
+         stream_handle(stream))
+    return out
+
+
+def detector(x: torch.Tensor, integration: int, out_pols: int, out: torch.Tensor | None = None,
+             accumulate=False, stream=None):
+    """Detector (Modules::Detector), CF32 [X,F,T,2] -> F32 [X,F,T/INTG,out_pols]."""
+    X, F, T, P = _shape4(x, "cf32")
+    if out is None:
+        out = torch.empty((X, F, T // integration, out_pols), dtype=torch.float32, device=x.device)
+    call("blade_detector", _ptr(x), _ptr(out), X, F, T, integration, out_pols, int(accumulate),
+         stream_handle(stream))
+    return out
+
+
+def gather(x: torch.Tensor, out: torch.Tensor, axis: int, multiplier: int, step: int,
+           type_name: str | None = None, stream=None):
+    """Place block ``step`` of ``multiplier`` along ``axis`` (Modules::Gather)."""
+    t = type_name or type_of(x)
+    shape = tuple(x.shape[:-1]) if t in ("ci8", "cf16") else tuple(x.shape)
+    esize = x.element_size() * (2 if t in ("ci8", "cf16") else 1)
+    height = 1
+    for d in shape[:axis]:
+        height *= d
+    width = esize
+    for d in shape[axis:]:
+        width *= d
+    call("blade_gather", _ptr(x), _ptr(out), height, width, multiplier, step, stream_handle(stream))
+    return out
+
+
+def permutation(x: torch.Tensor, indexes, type_name: str | None = None, stream=None):
+    """out = transpose(x, indexes) for a rank-4 tensor (Modules::Permutation)."""
+    t = type_name or type_of(x)
+    shape = tuple(x.shape[:-1]) if t in ("ci8", "cf16") else tuple(x.shape)
+    esize = x.element_size() * (2 if t in ("ci8", "cf16") else 1)
+    out_shape = tuple(shape[i] for i in indexes)
+    out = empty(out_shape, t, x.device)
+    s4 = (ct.c_uint64 * 4)(*shape)
+    i4 = (ct.c_uint64 * 4)(*indexes)
+    call("blade_permutation", _ptr(x), _ptr(out), esize, s4, i4, stream_handle(stream))
+    return out
+
+
+def phasor_ata(delays: torch.Tensor, calibration: torch.Tensor, observation_frequency_hz: float,
+               channel_bandwidth_hz: float, total_bandwidth_hz: float, frequency_start_index: int,
+               out_type="cf32", out: torch.Tensor | None = None, stream=None):
+    """ATA phasors from delays (Modules::Phasor::ATA's phasor loop), FP64 phase."""
+    B, A = delays.shape
+    A2, F, _, P = calibration.shape
+    if A2 != A or delays.dtype != torch.float64 or calibration.dtype != torch.complex128:
+        raise ValueError("delays must be F64 [B,A] and calibration CF64 [A,F,1,P]")
+    if out is None:
+        out = torch.empty((B, A, F, 1, P), dtype=_TORCH_OF[out_type], device=delays.device)
+    call("blade_phasor_ata", _ptr(delays), _ptr(calibration), _ptr(out), _lib.TYPE_CODES[out_type],
+         B, A, F, P, float(observation_frequency_hz), float(channel_bandwidth_hz),
+         float(total_bandwidth_hz), int(frequency_start_index), stream_handle(stream))
+    return out

@@ -1,0 +1,160 @@
+/*
+ * blade_amd.h — C-ABI of the MI355X (gfx950) ATA beamforming path.
+ *
+ * Plain pointers, sizes and a hipStream_t; no C++ or torch types.  Every
+ * buffer argument is DEVICE memory owned by the caller, and every call is
+ * stream-ordered and asynchronous.  Entry points return a blade_result
+ * (the reference's Result codes, include/blade/macros.hh:31-48);
+ * blade_amd_last_error() gives the calling thread's last message.
+ *
+ * Layouts are the reference's row-major tensors
+ * (include/blade/memory/custom.hh:10-163):
+ *   voltages / beams  [A|B, F, T, P]       (ArrayShape)
+ *   phasors           [B, A, F, 1, P]      (PhasorShape)
+ *   delays            [B, A]               (DelayShape)
+ * Complex types are interleaved {re, im}; CI8 is two int8.
+ */
+#ifndef BLADE_AMD_H
+#define BLADE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* blade_stream_t;   /* == hipStream_t */
+
+/* Result codes: values of the reference's Blade::Result (macros.hh:31-48). */
+enum blade_result {
+    BLADE_SUCCESS = 0,
+    BLADE_ERROR = 1,
+    BLADE_HIP_ERROR = 2,          /* the reference's CUDA_ERROR */
+    BLADE_ASSERTION_ERROR = 3,
+};
+
+/* Element types (the reference's TypeInfo names, include/blade/memory/types.hh). */
+enum blade_type {
+    BLADE_I8 = 0, BLADE_F16 = 1, BLADE_F32 = 2, BLADE_F64 = 3, BLADE_I32 = 4,
+    BLADE_CI8 = 5, BLADE_CF16 = 6, BLADE_CF32 = 7, BLADE_CF64 = 8,
+};
+
+/* Beamformer arithmetic. */
+enum blade_precision {
+    BLADE_PREC_F32 = 0,     /* FP32 MFMA (v_mfma_f32_*_f32), FP32 accumulate */
+    BLADE_PREC_F16X2 = 1,   /* split FP16: phasor = hi + lo, 2 FP16 MFMAs, FP32 accumulate (CI8 input only) */
+};
+
+/* Library identification; never fails. */
+const char* blade_amd_version(void);
+/* Thread-local message of the last failing call ("" if none). */
+const char* blade_amd_last_error(void);
+
+/*
+ * Cast — component-wise static_cast over n_elements tensor elements.
+ * Replaces Modules::Cast<IT,OT>::process (src/modules/cast/base.cc:60-66,
+ * kernel src/modules/cast/cast.cu:5-12).  Complex<->real pairs are rejected
+ * like cast/base.cc:39-43.  in == out is allowed only when the types match.
+ */
+int blade_cast(int in_type, int out_type, const void* in, void* out,
+               uint64_t n_elements, blade_stream_t stream);
+
+/*
+ * ATA beamformer.
+ * Replaces Modules::Beamformer::ATA<CF32,CF32> (src/modules/beamformer/ata.cc:7-72,
+ * generic.cc:31-33; kernel src/modules/beamformer/beamformer.cu:13-79):
+ *   y[b,f,t,p] = sum_a x[a,f,t,p] * w[b,a,f,0,p]
+ *   y[B,f,t,p] = (sum_a |x w[0]|^2, 0) when incoherent (sqrt'ed when incoherent_sqrt)
+ * input:  in_type CF32 or CI8, [A,F,T,P];  phasors: CF32 [B,A,F,1,P];
+ * output: CF32 [B + incoherent, F, T, P].  P must be 2 (as the reference).
+ * precision BLADE_PREC_F16X2 requires in_type == BLADE_CI8.
+ */
+int blade_beamformer_ata(int in_type, int precision, const void* input, const void* phasors,
+                         void* output, uint64_t A, uint64_t F, uint64_t T, uint64_t P,
+                         uint64_t B, int incoherent, int incoherent_sqrt,
+                         blade_stream_t stream);
+
+/*
+ * Fused ATA Mode-B chain on one block: beamform -> channelize (rate C in {1,2,4})
+ * -> optional polarizer (XY -> LR), one HBM pass.
+ * Equivalent to Beamformer::ATA (beamformer.cu:13-79) followed by
+ * Channelizer (oracle tests/modules/channelizer/advanced.py:49-65) and
+ * Polarizer (polarizer.cu:5-25); output CF32 [B + incoherent, F*C, T/C, P].
+ */
+int blade_beamform_chain(int in_type, int precision, const void* input, const void* phasors,
+                         void* output, uint64_t A, uint64_t F, uint64_t T, uint64_t P,
+                         uint64_t B, int incoherent, int incoherent_sqrt, uint64_t rate,
+                         int polarize, blade_stream_t stream);
+
+/*
+ * Channelizer — fftshift(FFT_C) of every C-sample chunk along T.
+ * Replaces Modules::Channelizer<CF32,CF32>::process (src/modules/channelizer/base.cc:122-146,
+ * pre-kernel channelizer.cu:5-19 and cuFFT C2C forward) with the general-rate
+ * semantics of the oracle tests/modules/channelizer/advanced.py:49-65:
+ *   out[a, f*C+k, s, p] = fftshift(fft(in[a, f, s*C:(s+1)*C, p]))[k]
+ * CF32 [A,F,T,P] -> CF32 [A,F*C,T/C,P]; rate 1 copies; C <= 16384.
+ * in == out (in-place) is supported.
+ */
+int blade_channelizer(const void* input, void* output, uint64_t A, uint64_t F, uint64_t T,
+                      uint64_t P, uint64_t rate, blade_stream_t stream);
+
+/*
+ * Polarizer XY -> LR: out[...,0] = X + jY, out[...,1] = X - jY.
+ * Replaces Modules::Polarizer<CF32|CF16>::process (src/modules/polarizer/base.cc:61-68,
+ * kernel polarizer.cu:5-25).  type: BLADE_CF32 or BLADE_CF16; n_elements = A*F*T*P
+ * (P = 2).  in == out allowed.
+ */
+int blade_polarizer(int type, const void* input, void* output, uint64_t n_elements,
+                    blade_stream_t stream);
+
+/*
+ * Detector — integrate `integration` consecutive samples of [X,F,T,2] CF32.
+ * Replaces Modules::Detector<CF32,F32>::process (src/modules/detector/base.cc:82-85,
+ * kernels detector.cu:7-51): out_pols 1 -> Stokes-I |X|^2+|Y|^2,
+ * out_pols 4 -> [XX*, YY*, Re XY*, Im XY*].  Output F32 [X, F, T/integration, out_pols].
+ * accumulate != 0 adds into `output` instead of overwriting it (stepped
+ * integration across blocks; the reference has none, detector.hh:11).
+ */
+int blade_detector(const void* input, void* output, uint64_t X, uint64_t F, uint64_t T,
+                   uint64_t integration, uint64_t out_pols, int accumulate,
+                   blade_stream_t stream);
+
+/*
+ * Gather — place one input block at position `step` of `multiplier` along an axis.
+ * Replaces Modules::Gather<T,T>::process (src/modules/gather/base.cc:184-225,
+ * kernel gather.cu:6-18 and Copy2D): with height = prod(shape[:axis]) and
+ * width_bytes = prod(shape[axis:]) * elem_bytes,
+ *   out[h, step*width_bytes + i] = in[h, i]   (out row pitch = width_bytes*multiplier).
+ * Bit-exact.
+ */
+int blade_gather(const void* input, void* output, uint64_t height, uint64_t width_bytes,
+                 uint64_t multiplier, uint64_t step, blade_stream_t stream);
+
+/*
+ * Permutation — out = transpose(in, axes=indexes) for a rank-4 tensor.
+ * Replaces Modules::Permutation<T,T>::process (src/modules/permutation/base.cc:319-322,
+ * kernel permutation.cu:6-20).  elem_bytes in {1,2,4,8,16}.  Bit-exact.
+ */
+int blade_permutation(const void* input, void* output, uint64_t elem_bytes,
+                      const uint64_t* in_shape4, const uint64_t* indexes4,
+                      blade_stream_t stream);
+
+/*
+ * ATA phasors from delays on the GPU, FP64 phase argument.
+ * Replaces the host loop of Modules::Phasor::ATA<OT>::process
+ * (src/modules/phasor/ata.cc:175-208):
+ *   w[b,a,f,p] = exp(-j2pi tau[b,a] ((f+fStart) dF + f_obs - BW/2)) * cal[a,f,0,p]
+ * delays: F64 [B,A] device; calibration: CF64 [A,F,1,P] device;
+ * out_type BLADE_CF32 or BLADE_CF64, output [B,A,F,1,P].
+ */
+int blade_phasor_ata(const void* delays, const void* calibration, void* phasors, int out_type,
+                     uint64_t B, uint64_t A, uint64_t F, uint64_t P,
+                     double observation_frequency_hz, double channel_bandwidth_hz,
+                     double total_bandwidth_hz, uint64_t frequency_start_index,
+                     blade_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BLADE_AMD_H */

@@ -1,0 +1,337 @@
+"""GPU parity of every kernel against the numpy oracle (through the C-ABI).
+
+Bars: FP32 paths <= 1e-5 normwise relative error against the complex128
+oracle; split-FP16 <= 1e-5 against the same FP64 oracle; Cast, Gather,
+Permutation and Polarizer bit-exact.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from blade_amd import ops
+from blade_amd._lib import BladeError
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def phasors_for(B, A, F, rng, tau_max=50e-9, f0=1.0e9, df=0.5e6, P=2):
+    tau = rng.uniform(-tau_max, tau_max, size=(B, A))
+    cal = np.ones((A, F, 1, P), dtype=np.complex128)
+    return O.phasor_ata(tau, cal, f0, df, 0.0, 0)
+
+
+def crand(rng, *shape):
+    return (rng.normal(size=shape) + 1j * rng.normal(size=shape)).astype(np.complex64)
+
+
+# ------------------------------------------------------------------ beamformer
+
+BF_CASES = [
+    # A,  F,   T,   B, inc, sqrt
+    (20, 192, 512, 8, True, True),     # config 0 shape
+    (2, 192, 512, 1, True, True),      # tests/modules/beamformer/ata.py
+    (28, 8, 8192, 16, False, False),   # config 1 per-channel shape
+    (28, 4, 16384, 8, True, False),
+    (42, 6, 2048, 4, True, True),      # config 4 shape
+    (1, 3, 64, 1, True, False),
+    (5, 3, 100, 9, True, True),        # B > 8 -> 32-row tile with padding
+    (27, 2, 1024, 33, True, False),    # three beam tiles
+    (14, 2, 250, 3, False, False),     # T % 4 != 0
+    (64, 2, 256, 64, True, True),
+]
+
+
+@pytest.mark.parametrize("A,F,T,B,inc,sq", BF_CASES)
+@pytest.mark.parametrize("in_type", ["cf32", "ci8"])
+def test_beamformer_fp32(A, F, T, B, inc, sq, in_type):
+    rng = np.random.default_rng(A * 1000 + B)
+    x8 = O.ci8_gaussian((A, F, T, 2), rng=rng)
+    w = phasors_for(B, A, F, rng)
+    ref = O.beamformer_ata(x8, w, inc, sq)
+    xin = dev(x8) if in_type == "ci8" else dev(O.ci8_to_complex(x8).astype(np.complex64))
+    y = host(ops.beamformer(xin, dev(w.astype(np.complex64)), inc, sq, precision="f32"))
+    err = O.norm_rel_err(y[:B], ref[:B])
+    assert err <= TOL, err
+    if inc:
+        assert O.norm_rel_err(y[B], ref[B]) <= TOL
+
+
+@pytest.mark.parametrize("A,F,T,B,inc,sq", BF_CASES)
+def test_beamformer_split_f16(A, F, T, B, inc, sq):
+    rng = np.random.default_rng(7 + A * 1000 + B)
+    x8 = O.ci8_gaussian((A, F, T, 2), rng=rng)
+    w = phasors_for(B, A, F, rng, tau_max=200e-9, f0=10e9)
+    ref = O.beamformer_ata(x8, w.astype(np.complex64), inc, sq)   # FP64 oracle of the CF32 phasors
+    y = host(ops.beamformer(dev(x8), dev(w.astype(np.complex64)), inc, sq, precision="f16x2"))
+    err = O.norm_rel_err(y[:B], ref[:B])
+    assert err <= TOL, err
+    if inc:
+        assert O.norm_rel_err(y[B], ref[B]) <= TOL
+
+
+def test_beamformer_random_phasors_reference_test():
+    # tests/modules/beamformer/ata.py: uniform random CF32 input and phasors.
+    rng = np.random.default_rng(5)
+    x = (rng.random((2, 192, 512, 2)) + 1j * rng.random((2, 192, 512, 2))).astype(np.complex64)
+    w = (rng.random((1, 2, 192, 1, 2)) + 1j * rng.random((1, 2, 192, 1, 2))).astype(np.complex64)
+    y = host(ops.beamformer(dev(x), dev(w), True, True))
+    ref = O.beamformer_ata(x, w, True, True)
+    assert O.norm_rel_err(y, ref) <= TOL
+
+
+@pytest.mark.parametrize("rate", [1, 2, 4])
+@pytest.mark.parametrize("polarize", [False, True])
+@pytest.mark.parametrize("precision", ["f32", "f16x2"])
+def test_beamform_chain(rate, polarize, precision):
+    rng = np.random.default_rng(rate * 10 + polarize)
+    A, F, T, B = 28, 6, 2048, 16
+    x8 = O.ci8_gaussian((A, F, T, 2), rng=rng)
+    w = phasors_for(B, A, F, rng).astype(np.complex64)
+    ref = O.beamformer_ata(x8, w, True, False)
+    ref = O.channelizer(ref, rate)
+    if polarize:
+        ref = O.polarizer(ref)
+    y = host(ops.beamform_chain(dev(x8), dev(w), True, False, rate, polarize, precision))
+    assert y.shape == ref.shape
+    assert O.norm_rel_err(y, ref) <= TOL
+
+
+def test_beamformer_rejects_bad_shapes():
+    x = torch.zeros((2, 2, 8, 2), dtype=torch.complex64, device="cuda")
+    w = torch.zeros((1, 2, 2, 1, 2), dtype=torch.complex64, device="cuda")
+    with pytest.raises(BladeError, match="CI8"):
+        ops.beamformer(x, w, precision="f16x2")
+    with pytest.raises(BladeError, match="rate"):
+        ops.beamform_chain(x, w, rate=8)
+
+
+# ----------------------------------------------------------------- channelizer
+
+CH_CASES = [
+    (5, 100, 200, 1, 200),     # tests/modules/channelizer/meson.build "weird"
+    (8, 128, 1024, 2, 1024),   # "hires"
+    (8, 128, 1024, 1, 1024),   # "hires-single-pol"
+    (4, 16, 8192, 2, 4),       # config 1 rate
+    (2, 4, 16384, 2, 8192),    # config 2 rate
+    (3, 5, 96, 2, 2),
+    (3, 5, 96, 2, 6),
+    (3, 5, 96, 2, 12),
+    (3, 5, 96, 2, 16),
+    (2, 3, 192, 2, 32),
+    (2, 3, 384, 2, 96),
+    (2, 3, 840, 1, 420),
+    (1, 2, 44, 2, 22),         # prime factor 11 -> direct DFT
+    (1, 2, 16384, 1, 16384),
+]
+
+
+@pytest.mark.parametrize("A,F,T,P,C", CH_CASES)
+def test_channelizer(A, F, T, P, C):
+    rng = np.random.default_rng(C)
+    x = crand(rng, A, F, T, P)
+    y = host(ops.channelizer(dev(x), C))
+    ref = O.channelizer(x, C)
+    assert y.shape == ref.shape
+    assert O.norm_rel_err(y, ref) <= TOL
+
+
+def test_channelizer_in_place_rate_equals_t():
+    rng = np.random.default_rng(3)
+    x = crand(rng, 2, 3, 1024, 2)
+    t = dev(x)
+    ops.channelizer(t, 1024, out=t)
+    assert O.norm_rel_err(host(t).reshape(2, 3 * 1024, 1, 2), O.channelizer(x, 1024)) <= TOL
+
+
+def test_channelizer_rate_one_is_copy():
+    rng = np.random.default_rng(4)
+    x = crand(rng, 2, 3, 16, 2)
+    assert np.array_equal(host(ops.channelizer(dev(x), 1)), x)
+
+
+# ------------------------------------------------------------------- polarizer
+
+def test_polarizer_cf32_bit_exact():
+    rng = np.random.default_rng(11)
+    x = crand(rng, 2, 192, 8750, 2)          # tests/modules/polarizer/polarizer.py shape
+    y = host(ops.polarizer(dev(x)))
+    X, Y = x[..., 0], x[..., 1]
+    exp = np.empty_like(x)
+    exp[..., 0] = (X.real - Y.imag) + 1j * (X.imag + Y.real)
+    exp[..., 1] = (X.real + Y.imag) + 1j * (X.imag - Y.real)
+    assert np.array_equal(y, exp)
+
+
+def test_polarizer_cf16_bit_exact():
+    rng = np.random.default_rng(12)
+    h = rng.normal(size=(2, 3, 101, 2, 2)).astype(np.float16)   # odd pair count -> tail path
+    y = host(ops.polarizer(dev(h)))
+    d = h.astype(np.float64)
+    xr, xi, yr, yi = d[..., 0, 0], d[..., 0, 1], d[..., 1, 0], d[..., 1, 1]
+    exp = np.stack([np.stack([xr - yi, xi + yr], -1), np.stack([xr + yi, xi - yr], -1)], -2)
+    assert np.array_equal(y, exp.astype(np.float16))
+
+
+# -------------------------------------------------------------------- detector
+
+@pytest.mark.parametrize("intg", [1, 4, 10, 16, 64, 128])
+@pytest.mark.parametrize("npol", [1, 4])
+def test_detector(intg, npol):
+    rng = np.random.default_rng(intg + npol)
+    T = 8750 if 8750 % intg == 0 else 8192
+    x = crand(rng, 2, 24, T, 2)
+    y = host(ops.detector(dev(x), intg, npol))
+    ref = O.detector(x, intg, npol)
+    assert O.norm_rel_err(y, ref) <= TOL
+
+
+def test_detector_stepped_accumulate():
+    rng = np.random.default_rng(21)
+    a, b = crand(rng, 2, 8, 64, 2), crand(rng, 2, 8, 64, 2)
+    out = torch.zeros((2, 8, 1, 1), dtype=torch.float32, device="cuda")
+    ops.detector(dev(a), 64, 1, out=out, accumulate=False)
+    ops.detector(dev(b), 64, 1, out=out, accumulate=True)
+    ref = O.detector_stepped([a, b], 128, 1)
+    assert O.norm_rel_err(host(out), ref) <= TOL
+
+
+# ------------------------------------------------------------------------ cast
+
+CAST_PAIRS = [("i8", "i8"), ("i8", "f32"), ("f16", "f16"), ("f16", "f32"), ("f32", "i32"),
+              ("f32", "f32"), ("f32", "f16"), ("ci8", "ci8"), ("ci8", "cf32"), ("ci8", "cf16"),
+              ("cf32", "cf16"), ("cf32", "cf32"), ("cf16", "cf32"), ("cf16", "cf16"),
+              ("f32", "f64"), ("cf32", "cf64"), ("f64", "f32"), ("i32", "f32")]
+
+
+def make_input(t, rng, n):
+    if t in ("i8", "ci8"):
+        v = rng.integers(-128, 128, size=(n, 2) if t == "ci8" else n).astype(np.int8)
+    elif t in ("f16", "cf16"):
+        v = (rng.normal(size=(n, 2) if t == "cf16" else n) * 300).astype(np.float16)
+    elif t == "f32":
+        v = (rng.normal(size=n) * 1e3).astype(np.float32)
+    elif t == "f64":
+        v = rng.normal(size=n) * 1e3
+    elif t == "i32":
+        v = rng.integers(-2**24, 2**24, size=n).astype(np.int32)
+    elif t == "cf32":
+        v = crand(rng, n) * 1e3
+    return v
+
+
+@pytest.mark.parametrize("it,ot", CAST_PAIRS)
+@pytest.mark.parametrize("n", [1, 7, 8, 1000, 100003])
+def test_cast_bit_exact(it, ot, n):
+    rng = np.random.default_rng(n)
+    v = make_input(it, rng, n)
+    y = host(ops.cast(dev(v), ot, in_type=it))
+    ref = O.cast(v, it, ot)
+    if ot in ("ci8", "cf16") and ref.ndim == 1:
+        ref = ref.reshape(-1, 2)
+    assert y.dtype == ref.dtype
+    assert np.array_equal(y.view(np.uint8), np.ascontiguousarray(ref).view(np.uint8))
+
+
+def test_cast_ci8_all_values():
+    v = np.arange(-128, 128, dtype=np.int8)
+    ci8 = np.stack(np.meshgrid(v, v, indexing="ij"), -1).reshape(-1, 2).copy()
+    y = host(ops.cast(dev(ci8), "cf32", in_type="ci8"))
+    assert np.array_equal(y.real, ci8[:, 0].astype(np.float32))
+    assert np.array_equal(y.imag, ci8[:, 1].astype(np.float32))
+
+
+# ---------------------------------------------------------------------- gather
+
+GATHER_CASES = [(2, 192, 8192, 2, 2, 16), (3, 7, 3, 1, 0, 5), (1, 2, 5, 6, 1, 6),
+                (11, 1, 13, 9, 2, 7), (9, 2, 1, 3, 3, 8), (9, 2, 5555, 3, 2, 8)]
+
+
+@pytest.mark.parametrize("A,F,T,P,axis,mult", GATHER_CASES)
+@pytest.mark.parametrize("dtype", ["cf32", "ci8"])
+def test_gather_bit_exact(A, F, T, P, axis, mult, dtype):
+    # tests/modules/gather/meson.build cases; blocks differ per step so a
+    # misplaced block is caught.
+    rng = np.random.default_rng(A + F + T)
+    shape = (A, F, T, P)
+    blocks = []
+    out_shape = list(shape)
+    out_shape[axis] *= mult
+    out = ops.empty(out_shape, dtype)
+    for step in range(mult):
+        if dtype == "cf32":
+            blk = crand(rng, *shape)
+        else:
+            blk = rng.integers(-128, 128, size=shape + (2,)).astype(np.int8)
+        blocks.append(blk)
+        ops.gather(dev(blk), out, axis, mult, step, type_name=dtype)
+    ref = O.gather(blocks, axis)
+    assert np.array_equal(host(out), ref)
+
+
+def test_gather_large_reference_case():
+    # (27, 131072, 3, 2) along axis 2 x 16 (tests/modules/gather/meson.build):
+    # 2.7 GB of output; compared on the device against torch.cat.
+    shape = (27, 131072, 3, 2)
+    out = torch.empty((27, 131072, 48, 2), dtype=torch.complex64, device="cuda")
+    blocks = []
+    g = torch.Generator(device="cuda").manual_seed(0)
+    for step in range(16):
+        blk = torch.randn(shape, dtype=torch.complex64, device="cuda", generator=g)
+        blocks.append(blk)
+        ops.gather(blk, out, 2, 16, step)
+    torch.cuda.synchronize()
+    assert torch.equal(out, torch.cat(blocks, dim=2))
+
+
+# ----------------------------------------------------------------- permutation
+
+@pytest.mark.parametrize("idx", [[0, 1, 2, 3], [0, 2, 1, 3], [3, 2, 1, 0], [2, 0, 3, 1],
+                                 [0, 1, 3, 2], [1, 0, 2, 3]])
+@pytest.mark.parametrize("dtype", ["cf32", "ci8", "cf16"])
+def test_permutation_bit_exact(idx, dtype):
+    rng = np.random.default_rng(sum(idx))
+    shape = (2, 192, 8192, 2) if dtype == "cf32" else (3, 17, 129, 2)
+    if dtype == "cf32":
+        x = crand(rng, *shape)
+        ref = O.permutation(x, idx)
+    elif dtype == "ci8":
+        x = rng.integers(-128, 128, size=shape + (2,)).astype(np.int8)
+        ref = np.ascontiguousarray(np.transpose(x, idx + [4]))
+    else:
+        x = rng.normal(size=shape + (2,)).astype(np.float16)
+        ref = np.ascontiguousarray(np.transpose(x, idx + [4]))
+    y = host(ops.permutation(dev(x), idx, type_name=dtype))
+    assert np.array_equal(y, ref)
+
+
+# ---------------------------------------------------------------------- phasor
+
+@pytest.mark.parametrize("B,A,F,tau_max,f_obs", [(2, 20, 192, 5e-6, 6500.125e6),
+                                                 (8, 28, 192, 200e-9, 10e9),
+                                                 (16, 28, 192, 50e-9, 1.0e9)])
+def test_phasor(B, A, F, tau_max, f_obs):
+    rng = np.random.default_rng(B)
+    tau = rng.uniform(-tau_max, tau_max, size=(B, A))
+    cal = rng.random((A, F, 1, 2)) + 1j * rng.random((A, F, 1, 2))
+    kw = dict(observation_frequency_hz=f_obs, channel_bandwidth_hz=0.5e6,
+              total_bandwidth_hz=1.024e9, frequency_start_index=352)
+    ref = O.phasor_ata(tau, cal, **kw)
+    w32 = host(ops.phasor_ata(dev(tau), dev(cal), out_type="cf32", **kw))
+    w64 = host(ops.phasor_ata(dev(tau), dev(cal), out_type="cf64", **kw))
+    scale = np.abs(cal).max()
+    assert np.max(np.abs(w32 - ref)) / scale < 2e-7
+    # the oracle's own FP64 phase carries ~1e-16 * |phase| of rounding
+    assert np.max(np.abs(w64 - ref)) / scale < 1e-10

@@ -1,0 +1,286 @@
+"""Pin the numpy oracle (CPU only).
+
+Every oracle function is checked against formulations that do not share its
+code: explicit loops, DFT matrices, closed forms and invariants, chosen so
+that a dropped term, a wrong sign/index or a transposed operand fails.
+"""
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+RNG = np.random.default_rng(1234)
+
+
+def crand(*shape):
+    return RNG.normal(size=shape) + 1j * RNG.normal(size=shape)
+
+
+# --------------------------------------------------------------------- phasor
+
+PH = dict(observation_frequency_hz=6500.125e6, channel_bandwidth_hz=0.5e6,
+          total_bandwidth_hz=1.024e9, frequency_start_index=352)
+
+
+def test_phasor_matches_reference_test_formulation():
+    # tests/modules/phasor/advanced.py:284-295,339-357 multiplies two
+    # exponentials; the module (ata.cc:186-204) exponentiates the sum.
+    tau = RNG.uniform(-5e-8, 5e-8, size=(3, 5))
+    cal = crand(5, 7, 1, 2)
+    a = O.phasor_ata(tau, cal, **PH)
+    b = O.phasor_ata_reference_test_form(tau, cal, **PH)
+    assert np.max(np.abs(a - b)) < 1e-9
+
+
+def test_phasor_special_cases():
+    cal = crand(4, 6, 1, 2)
+    # tau = 0 -> the calibration itself
+    w = O.phasor_ata(np.zeros((2, 4)), cal, **PH)
+    assert np.array_equal(w[0], cal) and np.array_equal(w[1], cal)
+    # cal = 1 -> unit modulus, phase linear in f with slope -2 pi tau dF
+    tau = np.array([[1e-8, -3e-9, 2.5e-8, 0.0]])
+    w = O.phasor_ata(tau, np.ones((4, 6, 1, 2)), **PH)
+    assert np.allclose(np.abs(w), 1.0, atol=1e-12)
+    ratio = w[0, :, 1:, 0, 0] / w[0, :, :-1, 0, 0]
+    expect = np.exp(-2j * np.pi * tau[0][:, None] * PH["channel_bandwidth_hz"])
+    assert np.allclose(ratio, expect, atol=1e-9)
+    # f = 0 phase: -2 pi tau (fStart dF + f_obs - BW/2)
+    f0 = PH["frequency_start_index"] * PH["channel_bandwidth_hz"] + \
+        PH["observation_frequency_hz"] - PH["total_bandwidth_hz"] / 2
+    assert np.allclose(w[0, :, 0, 0, 1], np.exp(-2j * np.pi * tau[0] * f0), atol=1e-9)
+
+
+# ----------------------------------------------------------------- beamformer
+
+def beamformer_loops(x, w, inc, sq):
+    A, F, T, P = x.shape
+    B = w.shape[0]
+    y = np.zeros((B + (1 if inc else 0), F, T, P), dtype=np.complex128)
+    for b in range(B):
+        for f in range(F):
+            for t in range(T):
+                for p in range(P):
+                    s = 0j
+                    for a in range(A):
+                        s += x[a, f, t, p] * w[b, a, f, 0, p]
+                    y[b, f, t, p] = s
+    if inc:
+        for f in range(F):
+            for t in range(T):
+                for p in range(P):
+                    s = 0.0
+                    for a in range(A):
+                        z = x[a, f, t, p] * w[0, a, f, 0, p]
+                        s += z.real * z.real + z.imag * z.imag   # detect() of beamformer.cu:9-11
+                    y[B, f, t, p] = np.sqrt(s) if sq else s
+    return y
+
+
+@pytest.mark.parametrize("inc,sq", [(False, False), (True, False), (True, True)])
+def test_beamformer_brute_force(inc, sq):
+    x = crand(3, 2, 5, 2)
+    w = crand(4, 3, 2, 1, 2)
+    assert np.allclose(O.beamformer_ata(x, w, inc, sq), beamformer_loops(x, w, inc, sq), atol=1e-12)
+
+
+def test_beamformer_invariants():
+    x = crand(5, 3, 8, 2)
+    w = crand(2, 5, 3, 1, 2)
+    # linearity in x and in w
+    x2 = crand(5, 3, 8, 2)
+    assert np.allclose(O.beamformer_ata(2 * x + x2, w), 2 * O.beamformer_ata(x, w) + O.beamformer_ata(x2, w))
+    w2 = crand(2, 5, 3, 1, 2)
+    assert np.allclose(O.beamformer_ata(x, w - 3j * w2), O.beamformer_ata(x, w) - 3j * O.beamformer_ata(x, w2))
+    # A = 1 -> x * w (no conjugation)
+    y = O.beamformer_ata(x[:1], w[:, :1])
+    assert np.allclose(y, x[0][None] * w[:, 0, :, :, :])
+    # w = 1 -> sum over antennas
+    y = O.beamformer_ata(x, np.ones((1, 5, 3, 1, 2)))
+    assert np.allclose(y[0], x.sum(0))
+    # incoherent beam uses beam 0 (beamformer.cu:68-69), not the last one
+    y = O.beamformer_ata(x, w, True, False)
+    expect = (np.abs(x) ** 2 * np.abs(w[0, :, :, 0, :])[:, :, None, :] ** 2).sum(0)
+    assert np.allclose(y[-1].real, expect) and np.all(y[-1].imag == 0)
+
+
+def test_ci8_widening_is_exact():
+    v = np.arange(-128, 128, dtype=np.int8)
+    z = O.ci8_to_complex(np.stack([v, v[::-1]], -1))
+    assert np.array_equal(z.real, v.astype(np.float64))
+    assert np.array_equal(z.imag, v[::-1].astype(np.float64))
+
+
+def test_ci8_generator_distribution():
+    x = O.ci8_gaussian((64, 64, 64), sigma=16, rng=0)
+    assert x.dtype == np.int8 and x.shape == (64, 64, 64, 2)
+    assert x.min() >= -127 and x.max() <= 127
+    assert abs(x.astype(float).std() - 16) < 0.2
+
+
+# ---------------------------------------------------------------- channelizer
+
+def channelizer_dft_matrix(x, C):
+    A, F, T, P = x.shape
+    S = T // C
+    n = np.arange(C)
+    M = np.exp(-2j * np.pi * np.outer(n, n) / C)       # X[k] = sum_n x[n] e^{-j2pi kn/C}
+    out = np.zeros((A, F * C, S, P), dtype=np.complex128)
+    for a in range(A):
+        for f in range(F):
+            for s in range(S):
+                for p in range(P):
+                    X = M @ x[a, f, s * C:(s + 1) * C, p]
+                    for k in range(C):
+                        out[a, f * C + (k + C // 2) % C, s, p] = X[k]
+    return out
+
+
+@pytest.mark.parametrize("C", [2, 4, 6, 8, 10, 16])
+def test_channelizer_dft_matrix(C):
+    x = crand(2, 3, 4 * C, 2)
+    assert np.allclose(O.channelizer(x, C), channelizer_dft_matrix(x, C), atol=1e-10)
+
+
+def test_channelizer_invariants():
+    C = 16
+    x = crand(1, 2, C * 3, 1)
+    y = O.channelizer(x, C)
+    # Parseval: sum |X|^2 = C sum |x|^2 (per spectrum, here over all)
+    assert np.isclose((np.abs(y) ** 2).sum(), C * (np.abs(x) ** 2).sum())
+    # impulse at n = 0 -> flat spectrum of ones
+    imp = np.zeros((1, 1, C, 1), complex)
+    imp[0, 0, 0, 0] = 1
+    assert np.allclose(O.channelizer(imp, C), 1.0)
+    # tone at bin k -> single bin at (k + C/2) mod C, amplitude C
+    for k in (0, 1, 5, C - 1):
+        tone = np.exp(2j * np.pi * k * np.arange(C) / C).reshape(1, 1, C, 1)
+        y = O.channelizer(tone, C)[0, :, 0, 0]
+        j = (k + C // 2) % C
+        assert np.isclose(y[j], C) and np.allclose(np.delete(y, j), 0, atol=1e-9)
+    # rate 1 is the identity
+    assert np.array_equal(O.channelizer(x, 1), x)
+
+
+# ------------------------------------------------------------------ polarizer
+
+def test_polarizer_closed_form():
+    x = crand(2, 3, 4, 2)
+    y = O.polarizer(x)
+    X, Y = x[..., 0], x[..., 1]
+    # explicit real arithmetic of polarizer.cu:18-21
+    assert np.allclose(y[..., 0].real, X.real - Y.imag) and np.allclose(y[..., 0].imag, X.imag + Y.real)
+    assert np.allclose(y[..., 1].real, X.real + Y.imag) and np.allclose(y[..., 1].imag, X.imag - Y.real)
+    # invertible: X = (L + R)/2, Y = (L - R)/(2j)
+    assert np.allclose((y[..., 0] + y[..., 1]) / 2, X)
+    assert np.allclose((y[..., 0] - y[..., 1]) / 2j, Y)
+    # bypass when input == output polarisation
+    assert np.array_equal(O.polarizer(x, "xy", "xy"), x)
+
+
+# ------------------------------------------------------------------- detector
+
+def detector_loops(x, n, npol):
+    Xn, F, T, _ = x.shape
+    out = np.zeros((Xn, F, T // n, npol))
+    for b in range(Xn):
+        for f in range(F):
+            for s in range(T // n):
+                X = x[b, f, s * n:(s + 1) * n, 0]
+                Y = x[b, f, s * n:(s + 1) * n, 1]
+                if npol == 1:
+                    out[b, f, s, 0] = np.sum(X.real ** 2 + X.imag ** 2) + np.sum(Y.real ** 2 + Y.imag ** 2)
+                else:
+                    c = X * np.conj(Y)
+                    out[b, f, s] = [np.sum((X * np.conj(X)).real), np.sum((Y * np.conj(Y)).real),
+                                    np.sum(c.real), np.sum(c.imag)]
+    return out
+
+
+@pytest.mark.parametrize("n,npol", [(1, 1), (10, 1), (1, 4), (10, 4), (4, 4)])
+def test_detector_brute_force(n, npol):
+    x = crand(2, 3, 20, 2)
+    assert np.allclose(O.detector(x, n, npol), detector_loops(x, n, npol))
+
+
+def test_detector_invariants():
+    x = crand(2, 3, 16, 2)
+    d4 = O.detector(x, 4, 4)
+    d1 = O.detector(x, 4, 1)
+    assert np.allclose(d4[..., 0] + d4[..., 1], d1[..., 0])
+    assert np.all(d4[..., 0] >= 0) and np.all(d4[..., 1] >= 0)
+    e = O.detector(x, 1, 4)
+    assert np.all(e[..., 2] ** 2 + e[..., 3] ** 2 <= e[..., 0] * e[..., 1] * (1 + 1e-12))
+    # stepped integration over two blocks == one block of twice the length
+    a, b = crand(1, 2, 8, 2), crand(1, 2, 8, 2)
+    assert np.allclose(O.detector_stepped([a, b], 16, 1), O.detector(np.concatenate([a, b], 2), 16, 1))
+    with pytest.raises(ValueError):
+        O.detector(x, 5, 1)
+
+
+# ----------------------------------------------------- cast/gather/permutation
+
+def test_cast_exhaustive_int8():
+    v = np.arange(-128, 128, dtype=np.int8)
+    assert np.array_equal(O.cast(v, "i8", "f32"), v.astype(np.float32))
+    ci8 = np.stack([v, v[::-1]], -1)
+    z = O.cast(ci8, "ci8", "cf32")
+    assert z.dtype == np.complex64
+    assert np.array_equal(z.real, v.astype(np.float32)) and np.array_equal(z.imag, v[::-1].astype(np.float32))
+    h = O.cast(ci8, "ci8", "cf16")
+    assert h.dtype == np.float16 and np.array_equal(h.astype(np.int32), ci8.astype(np.int32))
+
+
+def test_cast_float_rules():
+    f = np.array([1.5, -1.5, 2.7, -2.7, 65519.0, 1e-8, 3.14159], dtype=np.float32)
+    assert np.array_equal(O.cast(f, "f32", "i32"), np.array([1, -1, 2, -2, 65519, 0, 3], np.int32))
+    h = O.cast(f, "f32", "f16")
+    # round to nearest even, same as IEEE conversion
+    assert np.array_equal(h, f.astype(np.float16))
+    with pytest.raises(ValueError):
+        O.cast(f, "f32", "cf32")
+
+
+def test_gather_and_permutation():
+    blocks = [crand(2, 3, 4, 2) for _ in range(3)]
+    for axis in range(4):
+        g = O.gather(blocks, axis)
+        # block k occupies [k*n, (k+1)*n) along axis
+        n = blocks[0].shape[axis]
+        for k, blk in enumerate(blocks):
+            sl = [slice(None)] * 4
+            sl[axis] = slice(k * n, (k + 1) * n)
+            assert np.array_equal(g[tuple(sl)], blk)
+    x = crand(2, 3, 4, 5)
+    for idx in ([0, 1, 2, 3], [0, 2, 1, 3], [3, 2, 1, 0], [2, 0, 3, 1]):
+        y = O.permutation(x, idx)
+        # permutation.cu:13-17: out[permuted] = in[orig], permuted[d] = orig[idx[d]]
+        for c in np.ndindex(x.shape):
+            assert y[tuple(c[i] for i in idx)] == x[c]
+
+
+# -------------------------------------------------------------------- bundles
+
+def test_mode_b_composition():
+    x = crand(3, 2, 8, 2)
+    w = crand(2, 3, 2 * 4, 1, 2)
+    y = O.mode_b(x, w, channelizer_rate=4, convert_to_circular=True, incoherent_beam=True,
+                 detector_enable=True, detector_integration_size=2,
+                 detector_number_of_output_polarizations=1)
+    v = O.polarizer(O.channelizer(x, 4))
+    b = O.beamformer_ata(v, w, True, True)
+    assert np.allclose(y, O.detector(b, 2, 1))
+
+
+def test_calibration_gradient_closed_form():
+    x = crand(3, 2, 6, 2)
+    w = crand(1, 3, 2, 1, 2)
+    g = crand(3, 2, 2)
+    grad = O.stokes_i_loss_grad_gains(x, g, w)
+    h = 1e-6
+    for idx in [(0, 0, 0), (2, 1, 1), (1, 0, 1)]:
+        e = np.zeros_like(g)
+        e[idx] = h
+        dre = (O.beamformed_stokes_i_loss(x, g + e, w) - O.beamformed_stokes_i_loss(x, g - e, w)) / (2 * h)
+        dim = (O.beamformed_stokes_i_loss(x, g + 1j * e, w) - O.beamformed_stokes_i_loss(x, g - 1j * e, w)) / (2 * h)
+        assert np.isclose(grad[idx].real, dre, rtol=1e-6) and np.isclose(grad[idx].imag, dim, rtol=1e-6)
