@@ -33,6 +33,7 @@ SIGNATURES = {
     "blade_beamformer_ata": [_i, _i, _p, _p, _p, _u64, _u64, _u64, _u64, _u64, _i, _i, _p],
     "blade_beamform_chain": [_i, _i, _p, _p, _p, _u64, _u64, _u64, _u64, _u64, _i, _i, _u64, _i, _p],
     "blade_channelizer": [_p, _p, _u64, _u64, _u64, _u64, _u64, _p],
+    "blade_channelizer_plan": [_u64, _p],
     "blade_polarizer": [_i, _p, _p, _u64, _p],
     "blade_detector": [_p, _p, _u64, _u64, _u64, _u64, _u64, _i, _p],
     "blade_gather": [_p, _p, _u64, _u64, _u64, _u64, _p],

@@ -10,61 +10,44 @@
 // with M = 2B (rows: beam, re/im), K = 2A (antenna, re/im), N = T.  Phasors are
 // the A operand (staged once per workgroup in LDS), voltages the B operand,
 // read straight from HBM in their native [A,F,T,P] layout: each lane owns NT
-// consecutive time samples of one antenna, i.e. one 16-byte load (CI8, NT=4)
-// feeds 2 components x 2 pols x NT MFMAs.  The K index is ordered so that the
-// lane halves (32x32 tiles) or quarters (16x16 tiles) of the wavefront read
-// DIFFERENT antennas - no voltage is fetched twice.
+// consecutive time samples of one antenna, so one 16-byte load (CI8) feeds
+// 2 components x 2 pols x NT x MW MFMAs.  The K index is ordered so that the
+// four 16-lane groups of the wavefront read DIFFERENT antennas: no voltage is
+// fetched twice.
+//
+// 16x16 tiles are used throughout: on gfx950 v_mfma_f32_16x16x4_f32 and
+// v_mfma_f32_16x16x32_f16 issue at the full FP32 / FP16 matrix rate.  A
+// wavefront holds MW beam tiles (8 beams each) so that every voltage it loads
+// is multiplied by up to 16 beams.
 //
 // Two arithmetic variants:
-//  * FP32: v_mfma_f32_{32x32x2,16x16x4}_f32, FP32 accumulate.
+//  * FP32: v_mfma_f32_16x16x4_f32, FP32 accumulate.
 //  * split FP16 (CI8 input only): int8 -> fp16 is exact; each phasor
-//    component is scaled by a per-(tile,pol) power of two, split into
+//    component is scaled by a per-(workgroup, pol) power of two, split into
 //    hi = fp16(w) and lo = fp16(w - hi), and both halves are multiplied on
-//    v_mfma_f32_{32x32x16,16x16x32}_f16 into one FP32 accumulator; the power
-//    of two is removed exactly in the epilogue.
+//    v_mfma_f32_16x16x32_f16 into one FP32 accumulator; the power of two is
+//    removed exactly in the epilogue.
 //
-// The accumulator of a 32x32 tile holds, for lane l and register r,
-// row 8(r/4) + 4(l/32) + r%4 and column l%32; rows are (beam, re/im) pairs so
-// registers (r, r+1), r even, are one complex beam sample.  A 16x16 tile holds
-// row 4(l/16) + r, column l%16.
+// The accumulator of a 16x16 tile holds, for lane l and register r, row
+// 4(l/16) + r and column l%16; rows are (beam, re/im) pairs, so registers
+// (r, r+1), r even, hold one complex beam sample of time sample l%16.
 #include "common.hh"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace bl {
 
-template <int TILE>
-struct Acc;
-template <>
-struct Acc<32> {
-    typedef floatx16 type;
-    static constexpr int regs = 16;
-};
-template <>
-struct Acc<16> {
-    typedef floatx4 type;
-    static constexpr int regs = 4;
-};
+constexpr int TILE = 16;      // rows and columns of one MFMA tile
+constexpr int KG = 4;         // lane groups (l / 16) along K
+constexpr int WAVES = 4;      // wavefronts per workgroup
 
-template <int TILE>
-__device__ __forceinline__ typename Acc<TILE>::type mfma_f32(float a, float b,
-                                                             typename Acc<TILE>::type c) {
-    if constexpr (TILE == 32) return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
-    else return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+__device__ __forceinline__ floatx4 mfma_f32(float a, float b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-template <int TILE>
-__device__ __forceinline__ typename Acc<TILE>::type mfma_f16(halfx8 a, halfx8 b,
-                                                             typename Acc<TILE>::type c) {
-    if constexpr (TILE == 32) return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
-    else return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
-}
-
-// Row index of accumulator register r for a lane in k-group g.
-template <int TILE>
-__device__ __forceinline__ int acc_row(int r, int g) {
-    if constexpr (TILE == 32) return 8 * (r / 4) + 4 * g + (r % 4);
-    else return 4 * g + r;
+__device__ __forceinline__ floatx4 mfma_f16(halfx8 a, halfx8 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
 // Exact CI8 -> FP16 for one pol of a dword (re0, im0, re1, im1): the biased
@@ -91,90 +74,114 @@ struct BfArgs {
     int incoherent, inc_sqrt;
     int rate;               // epilogue channelizer rate (1, 2, 4)
     int polarize;
-    int nsteps;             // K steps
+    int nsteps;             // K steps (FP32: 2 per antenna group; FP16: 1 per 16 antennas)
+    int n_tchunks;          // chunks of WAVES*TILE*NT samples along T
+    int chunks_per_wg;      // consecutive chunks walked by one workgroup
 };
 
-// Epilogue: one complex beam sample for NT consecutive t, both pols, at
-// t0 = first time sample of the lane; writes plain or channelized output.
+__device__ __forceinline__ void polarize2(float2& X, float2& Y) {
+    const float2 L = make_float2(X.x - Y.y, X.y + Y.x);
+    const float2 R = make_float2(X.x + Y.y, X.y - Y.x);
+    X = L;
+    Y = R;
+}
+
+// Epilogue: beam b, both pols, NT consecutive time samples from t0; writes
+// the plain beam, or spectra of `rate` samples (fftshifted) into
+// [B', F*C, T/C, P], optionally polarized.
 template <int NT>
 __device__ __forceinline__ void store_beam(const BfArgs& a, int b, int f, int t0,
                                            float2 (&v)[2][NT]) {
     if (a.rate == 1) {
         if (a.polarize) {
 #pragma unroll
-            for (int n = 0; n < NT; n++) {
-                const float2 X = v[0][n], Y = v[1][n];
-                v[0][n] = make_float2(X.x - Y.y, X.y + Y.x);
-                v[1][n] = make_float2(X.x + Y.y, X.y - Y.x);
-            }
+            for (int n = 0; n < NT; n++) polarize2(v[0][n], v[1][n]);
         }
         float4* dst = reinterpret_cast<float4*>(a.y) + ((size_t)b * a.F + f) * a.T + t0;
 #pragma unroll
-        for (int n = 0; n < NT; n++)
-            dst[n] = make_float4(v[0][n].x, v[0][n].y, v[1][n].x, v[1][n].y);
+        for (int n = 0; n < NT; n++) dst[n] = make_float4(v[0][n].x, v[0][n].y, v[1][n].x, v[1][n].y);
         return;
     }
-    if constexpr (NT == 4) {
-        const int C = a.rate;
-        const int S = a.T / C;
-        // out[b, f*C + k', s, p]  with k' = (k + C/2) mod C
+    if constexpr (NT % 4 == 0) {
         float4* base = reinterpret_cast<float4*>(a.y);
-        if (C == 4) {
-            float2 o[2][4];
+        if (a.rate == 4) {
+            const int S = a.T / 4;
 #pragma unroll
-            for (int p = 0; p < 2; p++) {
-                const float2 s02 = cadd(v[p][0], v[p][2]), d02 = csub(v[p][0], v[p][2]);
-                const float2 s13 = cadd(v[p][1], v[p][3]);
-                const float2 dd = csub(v[p][1], v[p][3]);
-                const float2 d13 = make_float2(dd.y, -dd.x);   // -j (x1 - x3)
-                o[p][0] = cadd(s02, s13);
-                o[p][1] = cadd(d02, d13);
-                o[p][2] = csub(s02, s13);
-                o[p][3] = csub(d02, d13);
-            }
-            const int s = t0 / 4;
+            for (int h = 0; h < NT / 4; h++) {
+                float2 o[2][4];
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                float2 X = o[0][k], Y = o[1][k];
-                if (a.polarize) {
-                    const float2 L = make_float2(X.x - Y.y, X.y + Y.x);
-                    const float2 R = make_float2(X.x + Y.y, X.y - Y.x);
-                    X = L;
-                    Y = R;
+                for (int p = 0; p < 2; p++) {
+                    const float2* x = &v[p][4 * h];
+                    const float2 s02 = cadd(x[0], x[2]), d02 = csub(x[0], x[2]);
+                    const float2 s13 = cadd(x[1], x[3]), dd = csub(x[1], x[3]);
+                    const float2 d13 = make_float2(dd.y, -dd.x);   // -j (x1 - x3)
+                    o[p][0] = cadd(s02, s13);
+                    o[p][1] = cadd(d02, d13);
+                    o[p][2] = csub(s02, s13);
+                    o[p][3] = csub(d02, d13);
                 }
-                const int kp = (k + 2) & 3;
-                base[(((size_t)b * a.F + f) * 4 + kp) * S + s] = make_float4(X.x, X.y, Y.x, Y.y);
-            }
-        } else {   // C == 2: two spectra per lane
+                const int s = t0 / 4 + h;
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
+                for (int k = 0; k < 4; k++) {
+                    float2 X = o[0][k], Y = o[1][k];
+                    if (a.polarize) polarize2(X, Y);
+                    const int kp = (k + 2) & 3;   // fftshift
+                    base[(((size_t)b * a.F + f) * 4 + kp) * S + s] = make_float4(X.x, X.y, Y.x, Y.y);
+                }
+            }
+        } else {   // rate 2
+            const int S = a.T / 2;
+#pragma unroll
+            for (int h = 0; h < NT / 2; h++) {
                 const int s = t0 / 2 + h;
 #pragma unroll
                 for (int k = 0; k < 2; k++) {
                     float2 X = k == 0 ? cadd(v[0][2 * h], v[0][2 * h + 1]) : csub(v[0][2 * h], v[0][2 * h + 1]);
                     float2 Y = k == 0 ? cadd(v[1][2 * h], v[1][2 * h + 1]) : csub(v[1][2 * h], v[1][2 * h + 1]);
-                    if (a.polarize) {
-                        const float2 L = make_float2(X.x - Y.y, X.y + Y.x);
-                        const float2 R = make_float2(X.x + Y.y, X.y - Y.x);
-                        X = L;
-                        Y = R;
-                    }
-                    const int kp = k ^ 1;
-                    base[(((size_t)b * a.F + f) * 2 + kp) * S + s] = make_float4(X.x, X.y, Y.x, Y.y);
+                    if (a.polarize) polarize2(X, Y);
+                    base[(((size_t)b * a.F + f) * 2 + (k ^ 1)) * S + s] = make_float4(X.x, X.y, Y.x, Y.y);
                 }
             }
         }
     }
 }
 
-// LDS weight layout, FP32: wl[p][step][g][i] (i = row of the A operand).
-// LDS weight layout, split FP16: wh[p][step][hl][lane] halfx8.
-template <int TILE, int NT, bool CI8, bool F16X2>
+// One "unit" of voltage loads: what a lane needs for one K block.
+//  FP32, CI8  : one antenna x NT samples          -> NT words
+//  FP32, CF32 : one antenna x NT samples x 16 B   -> 4 NT words
+//  split FP16 : four antennas x NT samples        -> 4 NT words
+template <int NT, bool CI8, bool F16X2>
+struct Unit {
+    static constexpr int WORDS = (F16X2 || !CI8) ? 4 * NT : NT;
+    uint32_t w[WORDS];
+};
+
+template <int NW>
+__device__ __forceinline__ void load_words(uint32_t* dst, const uint32_t* src) {
+    if constexpr (NW % 4 == 0) {
+#pragma unroll
+        for (int k = 0; k < NW / 4; k++) {
+            const uint4 v = reinterpret_cast<const uint4*>(src)[k];
+            dst[4 * k] = v.x; dst[4 * k + 1] = v.y; dst[4 * k + 2] = v.z; dst[4 * k + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int n = 0; n < NW; n++) dst[n] = src[n];
+    }
+}
+
+// LDS phasor layouts (A operands):
+//   FP32       wl[m][p][step][g][row]        float   (step = 2 * antenna group + component)
+//   split FP16 wh[m][p][step][hl][lane]      halfx8  (8 k values = 4 antennas x re/im)
+//
+// Work split: blockIdx = (t-range, f, beam group of MW tiles).  A workgroup
+// walks `chunks_per_wg` consecutive chunks of WAVES*TILE*NT samples with the
+// phasors staged once; voltage loads run D units ahead of the MFMAs through
+// a register ring that crosses chunk boundaries.
+template <int NT, int MW, bool CI8, bool F16X2, int D>
 __global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
-    constexpr int KG = 64 / TILE;                 // k-groups of lanes
-    constexpr int AK = F16X2 ? 4 * KG : KG;       // antennas per K step (pair of steps for FP32)
-    constexpr int REGS = Acc<TILE>::regs;
-    typedef typename Acc<TILE>::type AccT;
+    constexpr int AK = F16X2 ? 4 * KG : KG;       // antennas per unit
+    typedef Unit<NT, CI8, F16X2> U_t;
 
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ unsigned int s_max[2];
@@ -185,27 +192,68 @@ __global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
     const int i = lane % TILE;     // A-operand row / accumulator column
     const int g = lane / TILE;     // k-group
     const int f = blockIdx.y;
-    const int mtile = blockIdx.z;
-    const int b_base = mtile * (TILE / 2);
+    const int b_wg = blockIdx.z * MW * (TILE / 2);
     const int nsteps = a.nsteps;
-    const bool do_inc = a.incoherent && mtile == 0;
+    const bool do_inc = a.incoherent && blockIdx.z == 0;
 
-    // ---- Stage phasors for this (f, beam tile) into LDS. -------------------
+    const int n_units = F16X2 ? nsteps : nsteps / 2;
+    const int upad = ((n_units + D - 1) / D) * D;
+    const int cw0 = blockIdx.x * a.chunks_per_wg;
+    const int nch = min(a.chunks_per_wg, a.n_tchunks - cw0);
+    const int J = nch * upad;
+    const size_t ant_stride = (size_t)a.F * a.T;   // complex samples between antennas
+
+    auto chunk_t0 = [&](int c) { return ((cw0 + c) * WAVES + wave) * (TILE * NT) + NT * i; };
+
+    // Loads are unconditional so that the wait-count analysis keeps them in
+    // flight across the loop: indices past the end re-load the last unit,
+    // padded antennas re-load antenna A-1 (their phasors are zero), and lanes
+    // past T re-load t = 0 (their results are not stored).
+    auto load_unit = [&](int j, U_t& u) {
+        j = min(j, J - 1);
+        const int c = j / upad;
+        const int un = min(j - c * upad, n_units - 1);
+        int t0 = chunk_t0(c);
+        t0 = (t0 < a.T) ? t0 : 0;
+        const size_t off = (size_t)f * a.T + t0;
+        if constexpr (F16X2) {
+            const uint32_t* x = static_cast<const uint32_t*>(a.x);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int an = min(un * AK + 4 * g + q, a.A - 1);
+                load_words<NT>(u.w + q * NT, x + an * ant_stride + off);
+            }
+        } else {
+            const int an = min(un * KG + g, a.A - 1);
+            if constexpr (CI8) {
+                load_words<NT>(u.w, static_cast<const uint32_t*>(a.x) + an * ant_stride + off);
+            } else {
+                load_words<4 * NT>(u.w, static_cast<const uint32_t*>(a.x) + 4 * (an * ant_stride + off));
+            }
+        }
+    };
+
+    // ---- Prime the voltage ring before staging phasors (overlaps latencies).
+    U_t ring[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) load_unit(d, ring[d]);
+
+    // ---- Stage phasors for this (f, beam group) into LDS. -------------------
     if (threadIdx.x < 2) s_max[threadIdx.x] = 0u;
     __syncthreads();
     if constexpr (F16X2) {
         // Per-pol power-of-two scale so that every scaled component is < 1.
-        float m[2] = {0.f, 0.f};
-        for (int e = threadIdx.x; e < (TILE / 2) * a.A; e += blockDim.x) {
-            const int bl_ = e / a.A, an = e - bl_ * a.A, b = b_base + bl_;
+        float mx[2] = {0.f, 0.f};
+        for (int e = threadIdx.x; e < MW * (TILE / 2) * a.A; e += blockDim.x) {
+            const int bl_ = e / a.A, an = e - bl_ * a.A, b = b_wg + bl_;
             if (b < a.B) {
                 const float4 w = reinterpret_cast<const float4*>(a.w)[((size_t)b * a.A + an) * a.F + f];
-                m[0] = fmaxf(m[0], fmaxf(fabsf(w.x), fabsf(w.y)));
-                m[1] = fmaxf(m[1], fmaxf(fabsf(w.z), fabsf(w.w)));
+                mx[0] = fmaxf(mx[0], fmaxf(fabsf(w.x), fabsf(w.y)));
+                mx[1] = fmaxf(mx[1], fmaxf(fabsf(w.z), fabsf(w.w)));
             }
         }
-        atomicMax(&s_max[0], __float_as_uint(m[0]));
-        atomicMax(&s_max[1], __float_as_uint(m[1]));
+        atomicMax(&s_max[0], __float_as_uint(mx[0]));
+        atomicMax(&s_max[1], __float_as_uint(mx[1]));
         __syncthreads();
         halfx8* wh = reinterpret_cast<halfx8*>(lds_raw);
         float scale[2];
@@ -215,15 +263,16 @@ __global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
             frexpf(__uint_as_float(s_max[p]), &ex);
             scale[p] = ldexpf(1.0f, -ex);
         }
-        // One halfx8 (8 k values) per (p, step, hl, lane).
-        const int total = 2 * nsteps * 2 * 64;
+        const int total = MW * 2 * nsteps * 2 * 64;
         for (int e = threadIdx.x; e < total; e += blockDim.x) {
             const int ln = e & 63;
             const int hl = (e >> 6) & 1;
-            const int st = (e >> 7) % nsteps;
-            const int p = (e >> 7) / nsteps;
+            const int r = e >> 7;                 // (m * 2 + p) * nsteps + st
+            const int st = r % nsteps;
+            const int p = (r / nsteps) & 1;
+            const int m = (r / nsteps) >> 1;
             const int row = ln % TILE, kg = ln / TILE;
-            const int b = b_base + row / 2, co = row & 1;
+            const int b = b_wg + m * (TILE / 2) + row / 2, co = row & 1;
             halfx8 h;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
@@ -235,8 +284,8 @@ __global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
                     wi = w.y * scale[p];
                 }
                 // k = (antenna, c): c = 0 multiplies Re x, c = 1 multiplies Im x.
-                const float v0 = co == 0 ? wr : wi;      // coefficient of Re x
-                const float v1 = co == 0 ? -wi : wr;     // coefficient of Im x
+                const float v0 = co == 0 ? wr : wi;
+                const float v1 = co == 0 ? -wi : wr;
                 const _Float16 h0 = (_Float16)v0, h1 = (_Float16)v1;
                 if (hl == 0) {
                     h[2 * q] = h0;
@@ -250,15 +299,16 @@ __global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
         }
     } else {
         float* wl = reinterpret_cast<float*>(lds_raw);
-        // nsteps = 2 * ceil(A / KG); step st = 2q + c; entry [p][st][kg][row]
-        const int total = 2 * nsteps * 64;
+        const int total = MW * 2 * nsteps * 64;
         for (int e = threadIdx.x; e < total; e += blockDim.x) {
             const int row = e % TILE;
             const int kg = (e / TILE) % KG;
-            const int st = (e / 64) % nsteps;
-            const int p = e / (64 * nsteps);
+            const int r = e / 64;                 // (m * 2 + p) * nsteps + st
+            const int st = r % nsteps;
+            const int p = (r / nsteps) & 1;
+            const int m = (r / nsteps) >> 1;
             const int an = (st / 2) * KG + kg, c = st & 1;
-            const int b = b_base + row / 2, co = row & 1;
+            const int b = b_wg + m * (TILE / 2) + row / 2, co = row & 1;
             float v = 0.f;
             if (b < a.B && an < a.A) {
                 const float2 w = a.w[(((size_t)b * a.A + an) * a.F + f) * 2 + p];
@@ -267,7 +317,7 @@ __global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
             wl[e] = v;
         }
     }
-    if (a.incoherent && mtile == 0) {
+    if (do_inc) {
         for (int e = threadIdx.x; e < 2 * a.A; e += blockDim.x) {
             const int an = e >> 1, p = e & 1;
             const float2 w = a.w[((size_t)an * a.F + f) * 2 + p];   // beam 0
@@ -276,146 +326,6 @@ __global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
     }
     __syncthreads();
 
-    // ---- Time samples owned by this lane. ----------------------------------
-    const int t_wave = (blockIdx.x * (blockDim.x / 64) + wave) * (TILE * NT);
-    const int t0 = t_wave + NT * i;
-    const bool t_ok = t0 < a.T;
-
-    AccT acc[2][NT];
-#pragma unroll
-    for (int p = 0; p < 2; p++)
-#pragma unroll
-        for (int n = 0; n < NT; n++)
-#pragma unroll
-            for (int r = 0; r < REGS; r++) acc[p][n][r] = 0.f;
-    float inc[2][NT];
-#pragma unroll
-    for (int p = 0; p < 2; p++)
-#pragma unroll
-        for (int n = 0; n < NT; n++) inc[p][n] = 0.f;
-
-    const size_t ant_stride = (size_t)a.F * a.T;   // samples between antennas
-    const size_t xoff = (size_t)f * a.T + t0;
-
-    if constexpr (F16X2) {
-        // K step st covers antennas st*AK + 4g + q, q = 0..3.
-        const uint32_t* x = static_cast<const uint32_t*>(a.x);
-        const halfx8* wh = reinterpret_cast<const halfx8*>(lds_raw);
-        for (int st = 0; st < nsteps; st++) {
-            uint32_t d[4][NT];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int an = st * AK + 4 * g + q;
-                if (t_ok && an < a.A) {
-                    const uint32_t* src = x + an * ant_stride + xoff;
-                    if constexpr (NT == 4) {
-                        const uint4 v = *reinterpret_cast<const uint4*>(src);
-                        d[q][0] = v.x; d[q][1] = v.y; d[q][2] = v.z; d[q][3] = v.w;
-                    } else {
-#pragma unroll
-                        for (int n = 0; n < NT; n++) d[q][n] = src[n];
-                    }
-                } else {
-#pragma unroll
-                    for (int n = 0; n < NT; n++) d[q][n] = 0x0u;
-                }
-            }
-#pragma unroll
-            for (int p = 0; p < 2; p++) {
-                const halfx8 whi = wh[((p * nsteps + st) * 2 + 0) * 64 + lane];
-                const halfx8 wlo = wh[((p * nsteps + st) * 2 + 1) * 64 + lane];
-#pragma unroll
-                for (int n = 0; n < NT; n++) {
-                    halfx8 xb;
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        const halfx2 h = ci8_pol_to_half2(d[q][n], p);
-                        xb[2 * q] = h[0];
-                        xb[2 * q + 1] = h[1];
-                    }
-                    acc[p][n] = mfma_f16<TILE>(whi, xb, acc[p][n]);
-                    acc[p][n] = mfma_f16<TILE>(wlo, xb, acc[p][n]);
-                }
-            }
-            if (do_inc) {
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int an = st * AK + 4 * g + q;
-                    if (an < a.A) {
-#pragma unroll
-                        for (int p = 0; p < 2; p++) {
-                            const float w2 = s_w0sq[p][an];
-#pragma unroll
-                            for (int n = 0; n < NT; n++) {
-                                const float xr = ci8_byte(d[q][n], 2 * p), xi = ci8_byte(d[q][n], 2 * p + 1);
-                                inc[p][n] += (xr * xr + xi * xi) * w2;
-                            }
-                        }
-                    }
-                }
-            }
-        }
-    } else {
-        const float* wl = reinterpret_cast<const float*>(lds_raw);
-        const int npairs = nsteps / 2;
-        for (int q = 0; q < npairs; q++) {
-            const int an = q * KG + g;
-            float xv[2][2][NT];   // [p][c][n]
-            if (t_ok && an < a.A) {
-                if constexpr (CI8) {
-                    const uint32_t* src = static_cast<const uint32_t*>(a.x) + an * ant_stride + xoff;
-                    uint32_t d[NT];
-                    if constexpr (NT == 4) {
-                        const uint4 v = *reinterpret_cast<const uint4*>(src);
-                        d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
-                    } else {
-#pragma unroll
-                        for (int n = 0; n < NT; n++) d[n] = src[n];
-                    }
-#pragma unroll
-                    for (int n = 0; n < NT; n++)
-#pragma unroll
-                        for (int p = 0; p < 2; p++)
-#pragma unroll
-                            for (int c = 0; c < 2; c++) xv[p][c][n] = ci8_byte(d[n], 2 * p + c);
-                } else {
-                    const float4* src = static_cast<const float4*>(a.x) + an * ant_stride + xoff;
-#pragma unroll
-                    for (int n = 0; n < NT; n++) {
-                        const float4 v = src[n];
-                        xv[0][0][n] = v.x; xv[0][1][n] = v.y; xv[1][0][n] = v.z; xv[1][1][n] = v.w;
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int n = 0; n < NT; n++)
-#pragma unroll
-                    for (int p = 0; p < 2; p++)
-#pragma unroll
-                        for (int c = 0; c < 2; c++) xv[p][c][n] = 0.f;
-            }
-#pragma unroll
-            for (int p = 0; p < 2; p++) {
-#pragma unroll
-                for (int c = 0; c < 2; c++) {
-                    const float wv = wl[((p * nsteps + 2 * q + c) * KG + g) * TILE + i];
-#pragma unroll
-                    for (int n = 0; n < NT; n++) acc[p][n] = mfma_f32<TILE>(wv, xv[p][c][n], acc[p][n]);
-                }
-            }
-            if (do_inc && an < a.A) {
-#pragma unroll
-                for (int p = 0; p < 2; p++) {
-                    const float w2 = s_w0sq[p][an];
-#pragma unroll
-                    for (int n = 0; n < NT; n++)
-                        inc[p][n] += (xv[p][0][n] * xv[p][0][n] + xv[p][1][n] * xv[p][1][n]) * w2;
-                }
-            }
-        }
-    }
-
-    // ---- Epilogue. ------------------------------------------------------------
     float unscale[2] = {1.f, 1.f};
     if constexpr (F16X2) {
 #pragma unroll
@@ -425,54 +335,202 @@ __global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
             unscale[p] = ldexpf(1.0f, ex);
         }
     }
-    if (t_ok) {
+
+    floatx4 acc[MW][2][NT];
+    float inc[2][NT];
+    auto zero_acc = [&]() {
 #pragma unroll
-        for (int r = 0; r < REGS; r += 2) {
-            const int row = acc_row<TILE>(r, g);
-            const int b = b_base + row / 2;
-            if (b < a.B) {
+        for (int m = 0; m < MW; m++)
+#pragma unroll
+            for (int p = 0; p < 2; p++)
+#pragma unroll
+                for (int n = 0; n < NT; n++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) acc[m][p][n][r] = 0.f;
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int n = 0; n < NT; n++) inc[p][n] = 0.f;
+    };
+    zero_acc();
+
+    // MFMAs (and incoherent power) of one unit.
+    auto compute = [&](const U_t& u, int un) {
+        if constexpr (F16X2) {
+            const halfx8* wh = reinterpret_cast<const halfx8*>(lds_raw);
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                halfx8 whi[MW], wlo[MW];
+#pragma unroll
+                for (int m = 0; m < MW; m++) {
+                    whi[m] = wh[(((m * 2 + p) * nsteps + un) * 2 + 0) * 64 + lane];
+                    wlo[m] = wh[(((m * 2 + p) * nsteps + un) * 2 + 1) * 64 + lane];
+                }
+#pragma unroll
+                for (int n = 0; n < NT; n++) {
+                    halfx8 xb;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const halfx2 h = ci8_pol_to_half2(u.w[q * NT + n], p);
+                        xb[2 * q] = h[0];
+                        xb[2 * q + 1] = h[1];
+                    }
+#pragma unroll
+                    for (int m = 0; m < MW; m++) {
+                        acc[m][p][n] = mfma_f16(whi[m], xb, acc[m][p][n]);
+                        acc[m][p][n] = mfma_f16(wlo[m], xb, acc[m][p][n]);
+                    }
+                }
+            }
+            if (do_inc) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int an = un * AK + 4 * g + q;
+                    const int ac = min(an, a.A - 1);
+#pragma unroll
+                    for (int p = 0; p < 2; p++) {
+                        const float w2 = an < a.A ? s_w0sq[p][ac] : 0.f;
+#pragma unroll
+                        for (int n = 0; n < NT; n++) {
+                            const float xr = ci8_byte(u.w[q * NT + n], 2 * p);
+                            const float xi = ci8_byte(u.w[q * NT + n], 2 * p + 1);
+                            inc[p][n] += (xr * xr + xi * xi) * w2;
+                        }
+                    }
+                }
+            }
+        } else {
+            const float* wl = reinterpret_cast<const float*>(lds_raw);
+            float xv[2][2][NT];   // [p][c][n]
+#pragma unroll
+            for (int n = 0; n < NT; n++) {
+                if constexpr (CI8) {
+#pragma unroll
+                    for (int p = 0; p < 2; p++)
+#pragma unroll
+                        for (int c = 0; c < 2; c++) xv[p][c][n] = ci8_byte(u.w[n], 2 * p + c);
+                } else {
+#pragma unroll
+                    for (int p = 0; p < 2; p++)
+#pragma unroll
+                        for (int c = 0; c < 2; c++) xv[p][c][n] = __uint_as_float(u.w[4 * n + 2 * p + c]);
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    float wv[MW];
+#pragma unroll
+                    for (int m = 0; m < MW; m++)
+                        wv[m] = wl[((((m * 2 + p) * nsteps) + 2 * un + c) * KG + g) * TILE + i];
+#pragma unroll
+                    for (int n = 0; n < NT; n++)
+#pragma unroll
+                        for (int m = 0; m < MW; m++) acc[m][p][n] = mfma_f32(wv[m], xv[p][c][n], acc[m][p][n]);
+                }
+            }
+            const int an = un * KG + g;
+            if (do_inc) {
+#pragma unroll
+                for (int p = 0; p < 2; p++) {
+                    const float w2 = an < a.A ? s_w0sq[p][min(an, a.A - 1)] : 0.f;
+#pragma unroll
+                    for (int n = 0; n < NT; n++)
+                        inc[p][n] += (xv[p][0][n] * xv[p][0][n] + xv[p][1][n] * xv[p][1][n]) * w2;
+                }
+            }
+        }
+    };
+
+    auto epilogue = [&](int c) {
+        const int t0 = chunk_t0(c);
+        const bool t_ok = t0 < a.T;
+        // Opaque copy of f: keeps the output addresses from being hoisted out
+        // of the chunk loop (they would pin dozens of registers).
+        int fo = f;
+        asm volatile("" : "+s"(fo));
+        if (t_ok) {
+#pragma unroll
+            for (int m = 0; m < MW; m++) {
+#pragma unroll
+                for (int r = 0; r < 4; r += 2) {
+                    const int b = b_wg + m * (TILE / 2) + (4 * g + r) / 2;
+                    if (b < a.B) {
+                        float2 v[2][NT];
+#pragma unroll
+                        for (int p = 0; p < 2; p++)
+#pragma unroll
+                            for (int n = 0; n < NT; n++)
+                                v[p][n] = make_float2(acc[m][p][n][r] * unscale[p], acc[m][p][n][r + 1] * unscale[p]);
+                        store_beam<NT>(a, b, fo, t0, v);
+                    }
+                }
+            }
+        }
+        if (do_inc) {
+#pragma unroll
+            for (int p = 0; p < 2; p++)
+#pragma unroll
+                for (int n = 0; n < NT; n++) {
+                    inc[p][n] += __shfl_xor(inc[p][n], 16, 64);
+                    inc[p][n] += __shfl_xor(inc[p][n], 32, 64);
+                }
+            if (t_ok && g == 0) {
                 float2 v[2][NT];
 #pragma unroll
                 for (int p = 0; p < 2; p++)
 #pragma unroll
                     for (int n = 0; n < NT; n++)
-                        v[p][n] = make_float2(acc[p][n][r] * unscale[p], acc[p][n][r + 1] * unscale[p]);
-                store_beam<NT>(a, b, f, t0, v);
+                        v[p][n] = make_float2(a.inc_sqrt ? sqrtf(inc[p][n]) : inc[p][n], 0.f);
+                store_beam<NT>(a, a.B, fo, t0, v);
             }
         }
-    }
-    if (do_inc) {
+    };
+
+    // ---- Main loop: D units per step, loads run D units ahead. -------------
+    int j = 0;   // global unit index (chunk-major)
+    for (int c = 0; c < nch; c++) {
+        for (int ub = 0; ub < upad; ub += D) {
 #pragma unroll
-        for (int p = 0; p < 2; p++)
-#pragma unroll
-            for (int n = 0; n < NT; n++) {
-#pragma unroll
-                for (int m = TILE; m < 64; m *= 2) inc[p][n] += __shfl_xor(inc[p][n], m, 64);
+            for (int d = 0; d < D; d++) {
+                const U_t cur = ring[d];
+                load_unit(j + d + D, ring[d]);
+                if (ub + d < n_units) compute(cur, ub + d);
             }
-        if (t_ok && g == 0) {
-            float2 v[2][NT];
-#pragma unroll
-            for (int p = 0; p < 2; p++)
-#pragma unroll
-                for (int n = 0; n < NT; n++)
-                    v[p][n] = make_float2(a.inc_sqrt ? sqrtf(inc[p][n]) : inc[p][n], 0.f);
-            store_beam<NT>(a, a.B, f, t0, v);
+            j += D;
         }
+        epilogue(c);
+        zero_acc();
     }
 }
 
-template <int TILE, int NT, bool CI8, bool F16X2>
+static int env_int(const char* name, int dflt) {
+    const char* v = getenv(name);
+    return v ? atoi(v) : dflt;
+}
+
+template <int NT, int MW, bool CI8, bool F16X2, int D>
 static int launch_bf(const BfArgs& args, hipStream_t s) {
-    constexpr int KG = 64 / TILE;
     BfArgs a = args;
     if (F16X2) a.nsteps = (a.A + 4 * KG - 1) / (4 * KG);
     else a.nsteps = 2 * ((a.A + KG - 1) / KG);
-    const size_t lds = F16X2 ? (size_t)2 * a.nsteps * 2 * 64 * 16 : (size_t)2 * a.nsteps * 64 * 4;
-    const int waves = 4;
-    const int span = waves * TILE * NT;
-    const int mtiles = (a.B + TILE / 2 - 1) / (TILE / 2);
-    dim3 grid((unsigned)((a.T + span - 1) / span), (unsigned)a.F, (unsigned)mtiles);
-    const void* fn = reinterpret_cast<const void*>(&beamform_kernel<TILE, NT, CI8, F16X2>);
+    const size_t lds = F16X2 ? (size_t)MW * 2 * a.nsteps * 2 * 64 * 16
+                             : (size_t)MW * 2 * a.nsteps * 64 * 4;
+    const int span = WAVES * TILE * NT;
+    const int groups = (a.B + MW * (TILE / 2) - 1) / (MW * (TILE / 2));
+    a.n_tchunks = (a.T + span - 1) / span;
+    // Chunks per workgroup: enough workgroups for ~3 rounds of 2 per CU.
+    int cpw = env_int("BLADE_BF_CHUNKS_PER_WG", 0);
+    if (cpw <= 0) {
+        const long target = 256L * 2 * 3;
+        const long per_chunk = (long)a.F * groups;
+        cpw = (int)std::max(1L, ((long)a.n_tchunks * per_chunk + target - 1) / target);
+        cpw = std::min(cpw, a.n_tchunks);
+    }
+    a.chunks_per_wg = cpw;
+    dim3 grid((unsigned)((a.n_tchunks + cpw - 1) / cpw), (unsigned)a.F, (unsigned)groups);
+    const void* fn = reinterpret_cast<const void*>(&beamform_kernel<NT, MW, CI8, F16X2, D>);
     if (lds > 150 * 1024) return fail("beamformer: too many antennas for the LDS phasor stage");
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -481,8 +539,24 @@ static int launch_bf(const BfArgs& args, hipStream_t s) {
             return BLADE_HIP_ERROR;
         }
     }
-    hipLaunchKernelGGL((beamform_kernel<TILE, NT, CI8, F16X2>), grid, dim3(64 * waves), lds, s, a);
+    hipLaunchKernelGGL((beamform_kernel<NT, MW, CI8, F16X2, D>), grid, dim3(64 * WAVES), lds, s, a);
     return check_launch("beamformer");
+}
+
+// Variant table: (time samples per lane, beam tiles per wave) by precision.
+template <bool CI8, bool F16X2>
+static int dispatch(const BfArgs& a, int nt, int mw, hipStream_t s) {
+    if constexpr (F16X2) {
+        if (nt == 4) return mw == 2 ? launch_bf<4, 2, true, true, 2>(a, s) : launch_bf<4, 1, true, true, 2>(a, s);
+        return mw == 2 ? launch_bf<1, 2, true, true, 2>(a, s) : launch_bf<1, 1, true, true, 2>(a, s);
+    } else if constexpr (CI8) {
+        if (nt == 8) return mw == 2 ? launch_bf<8, 2, true, false, 4>(a, s) : launch_bf<8, 1, true, false, 4>(a, s);
+        if (nt == 4) return mw == 2 ? launch_bf<4, 2, true, false, 4>(a, s) : launch_bf<4, 1, true, false, 4>(a, s);
+        return mw == 2 ? launch_bf<1, 2, true, false, 4>(a, s) : launch_bf<1, 1, true, false, 4>(a, s);
+    } else {
+        if (nt == 4) return mw == 2 ? launch_bf<4, 2, false, false, 2>(a, s) : launch_bf<4, 1, false, false, 2>(a, s);
+        return mw == 2 ? launch_bf<1, 2, false, false, 2>(a, s) : launch_bf<1, 1, false, false, 2>(a, s);
+    }
 }
 
 static int run_beamformer(int in_type, int precision, const void* input, const void* phasors,
@@ -503,6 +577,7 @@ static int run_beamformer(int in_type, int precision, const void* input, const v
     if (!input || !phasors || !output) return fail("beamformer: null pointer");
     if ((uintptr_t)input % 16 || (uintptr_t)phasors % 16 || (uintptr_t)output % 16)
         return fail("beamformer: buffers must be 16-byte aligned");
+    if (rate > 1 && T % 4) return fail("beamformer chain: T must be divisible by 4");
     BfArgs a;
     a.x = input;
     a.w = static_cast<const float2*>(phasors);
@@ -513,20 +588,17 @@ static int run_beamformer(int in_type, int precision, const void* input, const v
     a.rate = (int)rate;
     a.polarize = polarize;
     a.nsteps = 0;
-    const bool nt4 = (T % 4 == 0);
-    if (rate > 1 && !nt4) return fail("beamformer chain: T must be divisible by 4");
-    const bool small = B <= 8;   // M = 2B fits a 16-row tile
-    const bool ci8 = in_type == BLADE_CI8;
-    if (precision == BLADE_PREC_F16X2) {
-        if (small) return nt4 ? launch_bf<16, 4, true, true>(a, s) : launch_bf<16, 1, true, true>(a, s);
-        return nt4 ? launch_bf<32, 4, true, true>(a, s) : launch_bf<32, 1, true, true>(a, s);
+    a.n_tchunks = 0;
+    a.chunks_per_wg = 1;
+    const int mw = B > 8 ? 2 : 1;
+    if (precision == BLADE_PREC_F16X2) return dispatch<true, true>(a, T % 4 == 0 ? 4 : 1, mw, s);
+    if (in_type == BLADE_CI8) {
+        int nt = T % 8 == 0 ? 8 : (T % 4 == 0 ? 4 : 1);
+        const int want = env_int("BLADE_BF_NT", 0);
+        if (want > 0 && want < nt && T % want == 0 && (rate == 1 || want % 4 == 0)) nt = want;
+        return dispatch<true, false>(a, nt, mw, s);
     }
-    if (ci8) {
-        if (small) return nt4 ? launch_bf<16, 4, true, false>(a, s) : launch_bf<16, 1, true, false>(a, s);
-        return nt4 ? launch_bf<32, 4, true, false>(a, s) : launch_bf<32, 1, true, false>(a, s);
-    }
-    if (small) return nt4 ? launch_bf<16, 4, false, false>(a, s) : launch_bf<16, 1, false, false>(a, s);
-    return nt4 ? launch_bf<32, 4, false, false>(a, s) : launch_bf<32, 1, false, false>(a, s);
+    return dispatch<false, false>(a, T % 4 == 0 ? 4 : 1, mw, s);
 }
 
 }  // namespace bl

@@ -439,3 +439,11 @@ extern "C" int blade_channelizer(const void* input, void* output, uint64_t A, ui
                            in, out, C, S, n_spec, tw);
     return check_launch("channelizer");
 }
+
+extern "C" int blade_channelizer_plan(uint64_t rate, blade_stream_t stream) {
+    if (rate == 0 || (rate % 2 && rate != 1)) return fail("channelizer: invalid rate");
+    if (rate > (uint64_t)kMaxLdsComplex) return fail("channelizer: rate above 16384 is not supported");
+    if (rate <= 16) return BLADE_SUCCESS;   // register kernels carry their twiddles as arguments
+    const float2* tw = nullptr;
+    return get_twiddles((int)rate, reinterpret_cast<hipStream_t>(stream), &tw);
+}

@@ -1,0 +1,391 @@
+// Module implementations of the Blade C++ API over the gfx950 C-ABI
+// (blade_amd.h).  Constructors validate like the reference and throw a
+// Result; process() launches on the given stream and returns a Result.
+#include "blade/base.hh"
+
+#include <algorithm>
+#include <unordered_set>
+
+namespace Blade {
+namespace {
+
+// Streams passed to process() default to the null stream.
+inline blade_stream_t abi(const Stream& s) { return s.abi(); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- Beamformer
+// (reference src/modules/beamformer/generic.cc:9-33, ata.cc:7-72)
+
+namespace Modules::Beamformer {
+
+#undef BL_LOG_DOMAIN
+#define BL_LOG_DOMAIN "M::BEAMFORMER"
+
+template <typename IT, typename OT>
+Generic<IT, OT>::Generic(const Config& config, const Input& input, const Stream&)
+    : config(config), input(input) {
+    if (getInputBuffer().shape().numberOfTimeSamples() % config.blockSize != 0) {
+        BL_FATAL("Number of time samples ({}) isn't divisable by the block size ({}).",
+                 getInputBuffer().shape().numberOfTimeSamples(), config.blockSize);
+        throw Result::ERROR;
+    }
+    if (config.enableSplitFp16 && !std::is_same<IT, CI8>::value) {
+        BL_FATAL("Split-FP16 beamforming needs CI8 input (exact FP16 widening).");
+        throw Result::ERROR;
+    }
+    BL_INFO("Type: {} -> {}", TypeInfo<IT>::name, TypeInfo<OT>::name);
+    BL_INFO("Phasors Shape: {}", getInputPhasors().shape());
+    BL_INFO("Enable Incoherent Beam: {}", config.enableIncoherentBeam ? "YES" : "NO");
+    BL_INFO("Enable Incoherent Beam Square Root: {}", config.enableIncoherentBeamSqrt ? "YES" : "NO");
+    BL_INFO("Arithmetic: {}", config.enableSplitFp16 ? "split FP16 MFMA" : "FP32 MFMA");
+}
+
+template <typename IT, typename OT>
+Result Generic<IT, OT>::process(const U64&, const Stream& stream) {
+    const auto& s = getInputBuffer().shape();
+    BL_ABI_CHECK(blade_beamformer_ata(
+        TypeInfo<IT>::abi, config.enableSplitFp16 ? BLADE_PREC_F16X2 : BLADE_PREC_F32,
+        input.buf.data(), input.phasors.data(), output.buf.data(), s.numberOfAspects(),
+        s.numberOfFrequencyChannels(), s.numberOfTimeSamples(), s.numberOfPolarizations(),
+        getInputPhasors().shape().numberOfBeams(), config.enableIncoherentBeam,
+        config.enableIncoherentBeamSqrt, abi(stream)));
+    return Result::SUCCESS;
+}
+
+template <typename IT, typename OT>
+ATA<IT, OT>::ATA(const typename Generic<IT, OT>::Config& config,
+                 const typename Generic<IT, OT>::Input& input, const Stream& stream)
+    : Generic<IT, OT>(config, input, stream) {
+    const auto& ph = this->getInputPhasors().shape();
+    const auto& in = this->getInputBuffer().shape();
+    if (ph.numberOfBeams() > config.blockSize) {
+        BL_FATAL("The block size ({}) is smaller than the number of beams ({}).", config.blockSize,
+                 ph.numberOfBeams());
+        throw Result::ERROR;
+    }
+    if (ph.numberOfFrequencyChannels() != in.numberOfFrequencyChannels()) {
+        BL_FATAL("Number of frequency channels mismatch between phasors ({}) and buffer ({}).",
+                 ph.numberOfFrequencyChannels(), in.numberOfFrequencyChannels());
+        throw Result::ERROR;
+    }
+    if (ph.numberOfPolarizations() != in.numberOfPolarizations()) {
+        BL_FATAL("Number of polarizations mismatch between phasors ({}) and buffer ({}).",
+                 ph.numberOfPolarizations(), in.numberOfPolarizations());
+        throw Result::ERROR;
+    }
+    if (ph.numberOfAntennas() != in.numberOfAspects()) {
+        BL_FATAL("Number of antennas mismatch between phasors ({}) and buffer ({}).",
+                 ph.numberOfAntennas(), in.numberOfAspects());
+        throw Result::ERROR;
+    }
+    if (in.numberOfPolarizations() != 2) {
+        BL_FATAL("Number of polarizations ({}) must be two.", in.numberOfPolarizations());
+        throw Result::ERROR;
+    }
+    this->output.buf = ArrayTensor<Device::CUDA, OT>(this->getOutputBufferShape());
+    BL_INFO("Shape: {} -> {}", in, this->getOutputBuffer().shape());
+}
+
+template class BLADE_API Generic<CF32, CF32>;
+template class BLADE_API Generic<CI8, CF32>;
+template class BLADE_API ATA<CF32, CF32>;
+template class BLADE_API ATA<CI8, CF32>;
+
+}  // namespace Modules::Beamformer
+
+// --------------------------------------------------------------- Channelizer
+// (reference src/modules/channelizer/base.cc:9-146)
+
+namespace Modules {
+
+#undef BL_LOG_DOMAIN
+#define BL_LOG_DOMAIN "M::CHANNELIZER"
+
+template <typename IT, typename OT>
+Channelizer<IT, OT>::Channelizer(const Config& config, const Input& input, const Stream& stream)
+    : config(config), input(input), inPlace(false) {
+    const auto& s = getInputBuffer().shape();
+    if (config.rate == 0 || s.numberOfTimeSamples() % config.rate != 0) {
+        BL_FATAL("The number of time samples ({}) should be divisable by the channelizer rate ({}).",
+                 s.numberOfTimeSamples(), config.rate);
+        throw Result::ERROR;
+    }
+    if ((config.rate % 2) != 0 && config.rate != 1) {
+        BL_FATAL("The channelizer rate ({}) should be divisable by 2.", config.rate);
+        throw Result::ERROR;
+    }
+    if (s.numberOfPolarizations() != 2 && s.numberOfPolarizations() != 1) {
+        BL_FATAL("Number of polarizations ({}) of the input should be one or two.",
+                 s.numberOfPolarizations());
+        throw Result::ERROR;
+    }
+    // rate 1 and rate T keep the memory layout, so they run in place.
+    if (config.rate == 1 || config.rate == s.numberOfTimeSamples()) {
+        inPlace = true;
+        BL_CHECK_THROW(Link(output.buf, input.buf, getOutputBufferShape()));
+    } else {
+        output.buf = ArrayTensor<Device::CUDA, OT>(getOutputBufferShape());
+    }
+    if (config.rate > 1 && !Profiler::IsCapturing()) {
+        if (blade_channelizer_plan(config.rate, stream.abi()) != 0) {
+            BL_FATAL("{}", blade_amd_last_error());
+            throw Result::ERROR;
+        }
+    }
+    BL_INFO("Type: {} -> {}", TypeInfo<IT>::name, TypeInfo<OT>::name);
+    BL_INFO("Shape: {} -> {}", s, getOutputBuffer().shape());
+    BL_INFO("FFT Size: {}", config.rate);
+}
+
+template <typename IT, typename OT>
+Result Channelizer<IT, OT>::process(const U64&, const Stream& stream) {
+    if (config.rate == 1) return Result::SUCCESS;
+    const auto& s = getInputBuffer().shape();
+    BL_ABI_CHECK(blade_channelizer(input.buf.data(), output.buf.data(), s.numberOfAspects(),
+                                   s.numberOfFrequencyChannels(), s.numberOfTimeSamples(),
+                                   s.numberOfPolarizations(), config.rate, abi(stream)));
+    return Result::SUCCESS;
+}
+
+template class BLADE_API Channelizer<CF32, CF32>;
+
+// ----------------------------------------------------------------- Polarizer
+// (reference src/modules/polarizer/base.cc:12-71)
+
+#undef BL_LOG_DOMAIN
+#define BL_LOG_DOMAIN "M::POLARIZER"
+
+template <typename IT, typename OT>
+Polarizer<IT, OT>::Polarizer(const Config& config, const Input& input, const Stream&)
+    : config(config), input(input) {
+    if constexpr (!std::is_same<IT, OT>::value) {
+        BL_FATAL("This module requires the type of the input ({}) and output ({}) to be the same.",
+                 TypeInfo<IT>::name, TypeInfo<OT>::name);
+        throw Result::ERROR;
+    } else {
+        if (config.inputPolarization == config.outputPolarization) BL_INFO("Bypass: Enabled");
+        BL_CHECK_THROW(Link(output.buf, input.buf));
+        BL_INFO("Type: {} -> {}", TypeInfo<IT>::name, TypeInfo<OT>::name);
+        BL_INFO("Shape: {} -> {}", getInputBuffer().shape(), getOutputBuffer().shape());
+    }
+}
+
+template <typename IT, typename OT>
+Result Polarizer<IT, OT>::process(const U64&, const Stream& stream) {
+    if (config.inputPolarization == config.outputPolarization) return Result::SUCCESS;
+    BL_ABI_CHECK(blade_polarizer(TypeInfo<IT>::abi, input.buf.data(), output.buf.data(),
+                                 input.buf.size(), abi(stream)));
+    return Result::SUCCESS;
+}
+
+template class BLADE_API Polarizer<CF32, CF32>;
+template class BLADE_API Polarizer<CF16, CF16>;
+
+// ------------------------------------------------------------------ Detector
+// (reference src/modules/detector/base.cc:9-87)
+
+#undef BL_LOG_DOMAIN
+#define BL_LOG_DOMAIN "M::DETECTOR"
+
+template <typename IT, typename OT>
+Detector<IT, OT>::Detector(const Config& config, const Input& input, const Stream&)
+    : config(config), input(input), steps(1) {
+    const auto& s = getInputBuffer().shape();
+    if (config.integrationSize <= 0) {
+        BL_WARN("Integration size ({}) should be more than zero.", config.integrationSize);
+        throw Result::ERROR;
+    }
+    const U64 T = s.numberOfTimeSamples();
+    if (config.integrationSize > T && T > 0 && config.integrationSize % T == 0) {
+        steps = config.integrationSize / T;   // stepped integration across blocks
+    } else if (T % config.integrationSize != 0) {
+        BL_FATAL("The number of time samples ({}) should be divisable by the integration size ({}).",
+                 T, config.integrationSize);
+        throw Result::ERROR;
+    }
+    if (s.numberOfPolarizations() != 2) {
+        BL_FATAL("Number of polarizations ({}) should be two (2).", s.numberOfPolarizations());
+        throw Result::ERROR;
+    }
+    if (s.numberOfAspects() <= 0) {
+        BL_FATAL("Number of aspects ({}) should be more than zero.", s.numberOfAspects());
+        throw Result::ERROR;
+    }
+    if (config.numberOfOutputPolarizations != 4 && config.numberOfOutputPolarizations != 1) {
+        BL_FATAL("Number of output polarizations ({}) not supported.", config.numberOfOutputPolarizations);
+        throw Result::ERROR;
+    }
+    output.buf = ArrayTensor<Device::CUDA, OT>(getOutputBufferShape());
+    BL_INFO("Type: {} -> {}", TypeInfo<IT>::name, TypeInfo<OT>::name);
+    BL_INFO("Shape: {} -> {}", s, getOutputBuffer().shape());
+    BL_INFO("Integration Size: {} ({} block(s))", config.integrationSize, steps);
+    BL_INFO("Number of Output Polarizations: {}", config.numberOfOutputPolarizations);
+}
+
+template <typename IT, typename OT>
+Result Detector<IT, OT>::process(const U64& currentStepCount, const Stream& stream) {
+    const auto& s = getInputBuffer().shape();
+    const U64 T = s.numberOfTimeSamples();
+    const U64 intg = (steps > 1) ? T : config.integrationSize;
+    const int accumulate = (steps > 1 && currentStepCount > 0) ? 1 : 0;
+    BL_ABI_CHECK(blade_detector(input.buf.data(), output.buf.data(), s.numberOfAspects(),
+                                s.numberOfFrequencyChannels(), T, intg,
+                                config.numberOfOutputPolarizations, accumulate, abi(stream)));
+    return Result::SUCCESS;
+}
+
+template class BLADE_API Detector<CF32, F32>;
+
+// ---------------------------------------------------------------------- Cast
+// (reference src/modules/cast/base.cc:12-100)
+
+#undef BL_LOG_DOMAIN
+#define BL_LOG_DOMAIN "M::CAST"
+
+template <typename IT, typename OT>
+Cast<IT, OT>::Cast(const Config& config, const Input& input, const Stream&)
+    : config(config), input(input) {
+    if (TypeInfo<IT>::is_complex != TypeInfo<OT>::is_complex) {
+        BL_FATAL("Cannot cast between complex and non-complex types.");
+        throw Result::ERROR;
+    }
+    if constexpr (std::is_same<IT, OT>::value) {
+        BL_DEBUG("Bypassing cast because input and output types are the same.");
+        BL_CHECK_THROW(Link(output.buf, input.buf));
+    } else {
+        output.buf = ArrayTensor<Device::CUDA, OT>(getInputBuffer().shape());
+        BL_INFO("Type: {} -> {}", TypeInfo<IT>::name, TypeInfo<OT>::name);
+        BL_INFO("Shape: {} -> {}", getInputBuffer().shape(), getOutputBuffer().shape());
+    }
+}
+
+template <typename IT, typename OT>
+Result Cast<IT, OT>::process(const U64&, const Stream& stream) {
+    if constexpr (std::is_same<IT, OT>::value) {
+        return Result::SUCCESS;
+    } else {
+        BL_ABI_CHECK(blade_cast(TypeInfo<IT>::abi, TypeInfo<OT>::abi, input.buf.data(),
+                                output.buf.data(), input.buf.size(), abi(stream)));
+        return Result::SUCCESS;
+    }
+}
+
+template class BLADE_API Cast<I8, I8>;
+template class BLADE_API Cast<I8, F32>;
+template class BLADE_API Cast<F16, F16>;
+template class BLADE_API Cast<F16, F32>;
+template class BLADE_API Cast<F16, CF32>;
+template class BLADE_API Cast<F16, CF16>;
+template class BLADE_API Cast<F32, I32>;
+template class BLADE_API Cast<F32, F32>;
+template class BLADE_API Cast<F32, F16>;
+template class BLADE_API Cast<F32, CF32>;
+template class BLADE_API Cast<F32, CF16>;
+template class BLADE_API Cast<CI8, CI8>;
+template class BLADE_API Cast<CI8, CF32>;
+template class BLADE_API Cast<CI8, CF16>;
+template class BLADE_API Cast<CF32, F16>;
+template class BLADE_API Cast<CF32, F32>;
+template class BLADE_API Cast<CF32, CF16>;
+template class BLADE_API Cast<CF32, CF32>;
+template class BLADE_API Cast<CF16, F16>;
+template class BLADE_API Cast<CF16, F32>;
+template class BLADE_API Cast<CF16, CF32>;
+template class BLADE_API Cast<CF16, CF16>;
+
+// -------------------------------------------------------------------- Gather
+// (reference src/modules/gather/base.cc:12-116)
+
+#undef BL_LOG_DOMAIN
+#define BL_LOG_DOMAIN "M::GATHER"
+
+template <typename IT, typename OT>
+Gather<IT, OT>::Gather(const Config& config, const Input& input, const Stream&)
+    : config(config), input(input), computeRatio(config.multiplier) {
+    if constexpr (!std::is_same<IT, OT>::value) {
+        BL_FATAL("Input ({}) and output ({}) types aren't the same. Casting isn't supported by Gather yet.",
+                 TypeInfo<IT>::name, TypeInfo<OT>::name);
+        throw Result::ERROR;
+    }
+    if (config.axis >= input.buf.shape().dimensions()) {
+        BL_FATAL("Selected input axis ({}) is larger than input shape dimensions ({}).", config.axis,
+                 input.buf.shape().dimensions());
+        throw Result::ERROR;
+    }
+    if (config.multiplier <= 0) {
+        BL_FATAL("Multiplier ({}) should be more than zero.", config.multiplier);
+        throw Result::ERROR;
+    }
+    widthSize = 1;
+    for (U64 i = config.axis; i < input.buf.shape().dimensions(); i++) widthSize *= input.buf.shape()[i];
+    widthByteSize = widthSize * sizeof(IT);
+    heightSize = 1;
+    for (U64 i = 0; i < config.axis; i++) heightSize *= input.buf.shape()[i];
+    output.buf = ArrayTensor<Device::CUDA, OT>(getOutputBufferShape());
+    BL_INFO("Type: {} -> {}", TypeInfo<IT>::name, TypeInfo<OT>::name);
+    BL_INFO("Shape: {} -> {}", getInputBuffer().shape(), getOutputBuffer().shape());
+    BL_INFO("Axis: {}", config.axis);
+    BL_INFO("Multiplier: {}", computeRatio);
+}
+
+template <typename IT, typename OT>
+Result Gather<IT, OT>::process(const U64& currentStepCount, const Stream& stream) {
+    BL_ABI_CHECK(blade_gather(input.buf.data(), output.buf.data(), heightSize, widthByteSize,
+                              computeRatio, currentStepCount, abi(stream)));
+    return Result::SUCCESS;
+}
+
+template class BLADE_API Gather<CI8, CI8>;
+template class BLADE_API Gather<CF16, CF16>;
+template class BLADE_API Gather<CF32, CF32>;
+template class BLADE_API Gather<F32, F32>;
+
+// --------------------------------------------------------------- Permutation
+// (reference src/modules/permutation/base.cc:12-77)
+
+#undef BL_LOG_DOMAIN
+#define BL_LOG_DOMAIN "M::PERMUTATION"
+
+template <typename IT, typename OT>
+Permutation<IT, OT>::Permutation(const Config& config, const Input& input, const Stream&)
+    : config(config), input(input) {
+    if constexpr (!std::is_same<IT, OT>::value) {
+        BL_FATAL("Input ({}) and output ({}) types aren't the same. Casting isn't supported by Permutation yet.",
+                 TypeInfo<IT>::name, TypeInfo<OT>::name);
+        throw Result::ERROR;
+    }
+    std::unordered_set<U64> seen;
+    for (U64 i = 0; i < config.indexes.dimensions(); i++) {
+        if (config.indexes[i] >= 4) {
+            BL_FATAL("Permutation index {} is out of range.", config.indexes[i]);
+            throw Result::ERROR;
+        }
+        if (seen.count(config.indexes[i])) {
+            BL_FATAL("Repeated index {} in permutation.", config.indexes[i]);
+            throw Result::ERROR;
+        }
+        seen.insert(config.indexes[i]);
+    }
+    output.buf = ArrayTensor<Device::CUDA, OT>(getOutputBufferShape());
+    BL_INFO("Type: {} -> {}", TypeInfo<IT>::name, TypeInfo<OT>::name);
+    BL_INFO("Shape: {} -> {}", getInputBuffer().shape(), getOutputBuffer().shape());
+    BL_INFO("Indexes: {}", config.indexes);
+}
+
+template <typename IT, typename OT>
+Result Permutation<IT, OT>::process(const U64&, const Stream& stream) {
+    const auto& s = getInputBuffer().shape();
+    const uint64_t shape[4] = {s[0], s[1], s[2], s[3]};
+    const uint64_t idx[4] = {config.indexes[0], config.indexes[1], config.indexes[2], config.indexes[3]};
+    BL_ABI_CHECK(blade_permutation(input.buf.data(), output.buf.data(), sizeof(IT), shape, idx,
+                                   abi(stream)));
+    return Result::SUCCESS;
+}
+
+template class BLADE_API Permutation<CI8, CI8>;
+template class BLADE_API Permutation<CF16, CF16>;
+template class BLADE_API Permutation<CF32, CF32>;
+
+}  // namespace Modules
+}  // namespace Blade

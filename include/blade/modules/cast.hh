@@ -1,0 +1,46 @@
+// Cast module (reference include/blade/modules/cast.hh:9-70).
+#ifndef BLADE_MODULES_CAST_HH
+#define BLADE_MODULES_CAST_HH
+
+#include <string>
+
+#include "blade/module.hh"
+
+namespace Blade::Modules {
+
+template <typename IT, typename OT>
+class BLADE_API Cast : public Module {
+ public:
+    struct Config {
+        U64 blockSize = 512;
+    };
+
+    const Config& getConfig() const { return config; }
+
+    struct Input {
+        const ArrayTensor<Device::CUDA, IT>& buf;
+    };
+
+    const ArrayTensor<Device::CUDA, IT>& getInputBuffer() const { return input.buf; }
+
+    struct Output {
+        ArrayTensor<Device::CUDA, OT> buf;
+    };
+
+    const ArrayTensor<Device::CUDA, OT>& getOutputBuffer() const { return output.buf; }
+
+    Taint getTaint() const override { return Taint::CONSUMER | Taint::PRODUCER; }
+    std::string name() const override { return "Cast"; }
+
+    explicit Cast(const Config& config, const Input& input, const Stream& stream = {});
+    Result process(const U64& currentStepCount, const Stream& stream = {}) override;
+
+ private:
+    const Config config;
+    const Input input;
+    Output output;
+};
+
+}  // namespace Blade::Modules
+
+#endif

@@ -99,6 +99,14 @@ int blade_channelizer(const void* input, void* output, uint64_t A, uint64_t F, u
                       uint64_t P, uint64_t rate, blade_stream_t stream);
 
 /*
+ * Build (once per device and rate) the twiddle table blade_channelizer uses,
+ * so that later calls allocate nothing (e.g. inside hipGraph capture).
+ * Replaces the cuFFT plan creation of the Channelizer constructor
+ * (src/modules/channelizer/base.cc:58-93).
+ */
+int blade_channelizer_plan(uint64_t rate, blade_stream_t stream);
+
+/*
  * Polarizer XY -> LR: out[...,0] = X + jY, out[...,1] = X - jY.
  * Replaces Modules::Polarizer<CF32|CF16>::process (src/modules/polarizer/base.cc:61-68,
  * kernel polarizer.cu:5-25).  type: BLADE_CF32 or BLADE_CF16; n_elements = A*F*T*P

@@ -23,7 +23,7 @@ __all__ = [
     "phasor_ata", "phasor_ata_reference_test_form",
     "beamformer_ata", "channelizer", "polarizer", "detector",
     "detector_stepped", "cast", "gather", "permutation",
-    "mode_b", "mode_h", "norm_rel_err", "max_rms_rel_err",
+    "mode_b", "mode_h", "beamform_chain_c64", "norm_rel_err", "max_rms_rel_err",
     "beamformed_stokes_i_loss", "stokes_i_loss_grad_gains",
     "NUMPY_TYPES",
 ]
@@ -333,6 +333,40 @@ def mode_h(x, convert_to_circular=False, detector_integration_size=1,
     v = channelizer(x, x.shape[2])
     v = polarizer(v, "xy", "lr" if convert_to_circular else "xy")
     return detector(v, detector_integration_size, detector_number_of_output_polarizations)
+
+
+def beamform_chain_c64(x, phasors, rate=1, polarize=False):
+    """Vectorised complex64 CPU port of beamform -> channelize -> [polarize].
+
+    The same composition as ``beamformer_ata`` (``beamformer.cu:13-79``) ->
+    ``channelizer`` (``advanced.py:49-65``) -> ``polarizer``
+    (``polarizer.py:47-49``) without the incoherent beam, written for CPU
+    speed (batched BLAS cgemm over (f, p), numpy FFT) in the precision of the
+    GPU's FP32 path.  It is the timed CPU baseline; it is pinned against the
+    complex128 functions above by ``tests/test_oracle.py``.
+    """
+    x = np.asarray(x)
+    if x.dtype == np.int8:
+        x = (x[..., 0].astype(np.float32) + 1j * x[..., 1].astype(np.float32)).astype(np.complex64)
+    x = x.astype(np.complex64, copy=False)
+    A, F, T, P = x.shape
+    w = np.asarray(phasors).astype(np.complex64)[:, :, :, 0, :]                # [B, A, F, P]
+    B = w.shape[0]
+    xt = np.ascontiguousarray(x.transpose(1, 3, 0, 2))                          # [F, P, A, T]
+    wt = np.ascontiguousarray(w.transpose(2, 3, 0, 1))                          # [F, P, B, A]
+    y = np.ascontiguousarray(np.matmul(wt, xt).transpose(2, 0, 3, 1))           # [B, F, T, P]
+    C = int(rate)
+    if C > 1:
+        S = T // C
+        spec = np.fft.fftshift(np.fft.fft(y.reshape(B, F, S, C, P), axis=3), axes=3)
+        y = np.ascontiguousarray(spec.transpose(0, 1, 3, 2, 4)).reshape(B, F * C, S, P)
+        y = y.astype(np.complex64, copy=False)
+    if polarize:
+        o = np.empty_like(y)
+        o[..., 0] = y[..., 0] + 1j * y[..., 1]
+        o[..., 1] = y[..., 0] - 1j * y[..., 1]
+        y = o
+    return y
 
 
 # ---------------------------------------------------------------------------

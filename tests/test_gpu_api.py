@@ -1,0 +1,427 @@
+"""The reference's own module tests, through the drop-in Python API.
+
+Each test follows the structure of the reference test it cites
+(``@bl.runner`` pipeline, ``bl.module``, host tensors filled through
+``as_numpy``) with seeded inputs, and checks against the numpy oracle at this
+build's bar (1e-5 normwise for FP32, bit-exact for data movement) instead of
+the reference's rtol 0.01-0.5.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+import blade_amd as bl
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def fill(t, values):
+    np.copyto(t.as_numpy(), values)
+
+
+def crand(rng, shape):
+    return (rng.random(size=shape) + 1j * rng.random(size=shape)).astype(np.complex64)
+
+
+# ------------------------------------------------------- beamformer (ata.py)
+
+@bl.runner
+class BeamformerPipeline:
+    def __init__(self, input_shape, phasor_shape, output_shape, config):
+        self.input.buf = bl.array_tensor(input_shape, dtype=bl.cf32)
+        self.input.phasors = bl.phasor_tensor(phasor_shape, dtype=bl.cf32)
+        self.output.buf = bl.array_tensor(output_shape, dtype=bl.cf32)
+        input = (self.input.buf, self.input.phasors)
+        self.module.beamformer = bl.module(bl.beamformer, config, input, telescope=bl.ata)
+
+    def transfer_in(self, buf, phasors):
+        self.copy(self.input.buf, buf)
+        self.copy(self.input.phasors, phasors)
+
+    def transfer_out(self, buf):
+        self.copy(self.output.buf, self.module.beamformer.get_output())
+        self.copy(buf, self.output.buf)
+
+
+@pytest.mark.parametrize("shapes", [((2, 192, 512, 2), (1, 2, 192, 1, 2)),
+                                    ((20, 192, 512, 2), (8, 20, 192, 1, 2))])
+def test_beamformer_module(shapes):
+    # tests/modules/beamformer/ata.py
+    input_shape, phasor_shape = shapes
+    B = phasor_shape[0]
+    output_shape = (B + 1,) + input_shape[1:]
+    config = {"enable_incoherent_beam": True, "enable_incoherent_beam_sqrt": True}
+    rng = np.random.default_rng(0)
+    host_input = bl.array_tensor(input_shape, dtype=bl.cf32, device=bl.cpu)
+    host_phasors = bl.phasor_tensor(phasor_shape, dtype=bl.cf32, device=bl.cpu)
+    host_output = bl.array_tensor(output_shape, dtype=bl.cf32, device=bl.cpu)
+    fill(host_input, crand(rng, input_shape))
+    fill(host_phasors, crand(rng, phasor_shape))
+    pipeline = BeamformerPipeline(input_shape, phasor_shape, output_shape, config)
+    pipeline(host_input, host_phasors, host_output)
+    ref = O.beamformer_ata(host_input.as_numpy(), host_phasors.as_numpy(), True, True)
+    assert O.norm_rel_err(host_output.as_numpy(), ref) <= TOL
+
+
+def test_beamformer_module_ci8_split_fp16():
+    # The CI8 instantiation (widening fused into the kernel) with split FP16.
+    A, F, T, B = 28, 8, 1024, 16
+    rng = np.random.default_rng(1)
+    x8 = O.ci8_gaussian((A, F, T, 2), rng=rng)
+    w = O.phasor_ata(rng.uniform(-5e-8, 5e-8, (B, A)), np.ones((A, F, 1, 2)), 1e9, 0.5e6, 0, 0)
+    xin = bl.array_tensor((A, F, T, 2), dtype=bl.ci8)
+    ph = bl.phasor_tensor((B, A, F, 1, 2), dtype=bl.cf32)
+    hx = bl.array_tensor((A, F, T, 2), dtype=bl.ci8, device=bl.cpu)
+    hw = bl.phasor_tensor((B, A, F, 1, 2), dtype=bl.cf32, device=bl.cpu)
+    fill(hx, x8)
+    fill(hw, w.astype(np.complex64))
+    bl.copy(xin, hx)
+    bl.copy(ph, hw)
+    mod = bl.module(bl.beamformer, {"enable_incoherent_beam": False, "enable_split_fp16": True},
+                    (xin, ph), it=bl.ci8, telescope=bl.ata)
+    assert mod.process(0) == bl.result.success
+    out = bl.array_tensor((B, F, T, 2), dtype=bl.cf32, device=bl.cpu)
+    bl.copy(out, mod.get_output())
+    assert O.norm_rel_err(out.as_numpy(), O.beamformer_ata(x8, w.astype(np.complex64))) <= TOL
+
+
+# ---------------------------------------------------- channelizer (advanced.py)
+
+@bl.runner
+class ChannelizerPipeline:
+    def __init__(self, input_shape, output_shape, config):
+        self.input.buf = bl.array_tensor(input_shape, dtype=bl.cf32)
+        self.output.buf = bl.array_tensor(output_shape, dtype=bl.cf32)
+        self.module.channelizer = bl.module(bl.channelizer, config, self.input.buf)
+
+    def transfer_in(self, buf):
+        self.copy(self.input.buf, buf)
+
+    def transfer_out(self, buf):
+        self.copy(self.output.buf, self.module.channelizer.get_output())
+        self.copy(buf, self.output.buf)
+
+
+@pytest.mark.parametrize("A,F,T,P,C", [(5, 100, 200, 1, 200), (8, 128, 1024, 2, 1024),
+                                       (8, 128, 1024, 1, 1024), (4, 16, 8192, 2, 4)])
+def test_channelizer_module(A, F, T, P, C):
+    input_shape = (A, F, T, P)
+    output_shape = (A, F * C, T // C, P)
+    rng = np.random.default_rng(C)
+    host_input = bl.array_tensor(input_shape, dtype=bl.cf32, device=bl.cpu)
+    host_output = bl.array_tensor(output_shape, dtype=bl.cf32, device=bl.cpu)
+    fill(host_input, crand(rng, input_shape))
+    pipeline = ChannelizerPipeline(input_shape, output_shape, {"rate": C})
+    pipeline(host_input, host_output)
+    ref = O.channelizer(host_input.as_numpy(), C)
+    assert O.norm_rel_err(host_output.as_numpy(), ref) <= TOL
+
+
+@pytest.mark.parametrize("A,F,T,P,C", [(8, 128, 1024, 1, 512 + 1), (8, 128, 1025, 1, 1025)])
+def test_channelizer_module_rejects(A, F, T, P, C):
+    # tests/modules/channelizer/meson.build fail cases (odd rate / T % rate)
+    with pytest.raises(RuntimeError):
+        ChannelizerPipeline((A, F, T, P), (A, F * C, T // C, P), {"rate": C})
+
+
+# ------------------------------------------------------ polarizer (polarizer.py)
+
+@bl.runner
+class PolarizerPipeline:
+    def __init__(self, shape, config):
+        self.input.buf = bl.array_tensor(shape, dtype=bl.cf32)
+        self.output.buf = bl.array_tensor(shape, dtype=bl.cf32)
+        self.module.polarizer = bl.module(bl.polarizer, config, self.input.buf)
+
+    def transfer_in(self, buf):
+        self.copy(self.input.buf, buf)
+
+    def transfer_out(self, buf):
+        self.copy(self.output.buf, self.module.polarizer.get_output())
+        self.copy(buf, self.output.buf)
+
+
+def test_polarizer_module():
+    shape = (2, 192, 8750, 2)
+    rng = np.random.default_rng(2)
+    host_input = bl.array_tensor(shape, dtype=bl.cf32, device=bl.cpu)
+    host_output = bl.array_tensor(shape, dtype=bl.cf32, device=bl.cpu)
+    fill(host_input, crand(rng, shape))
+    pipeline = PolarizerPipeline(shape, {"inputPolarization": bl.pol.xy, "outputPolarization": bl.pol.lr})
+    pipeline(host_input, host_output)
+    x = host_input.as_numpy()
+    exp = np.empty_like(x)
+    exp[..., 0] = (x[..., 0].real - x[..., 1].imag) + 1j * (x[..., 0].imag + x[..., 1].real)
+    exp[..., 1] = (x[..., 0].real + x[..., 1].imag) + 1j * (x[..., 0].imag - x[..., 1].real)
+    assert np.array_equal(host_output.as_numpy(), exp)
+
+
+# ------------------------------------------- detector (single_pol / cross_pol)
+
+@bl.runner
+class DetectorPipeline:
+    def __init__(self, input_shape, output_shape, config):
+        self.input.buf = bl.array_tensor(input_shape, dtype=bl.cf32)
+        self.output.buf = bl.array_tensor(output_shape, dtype=bl.f32)
+        self.module.detector = bl.module(bl.detector, config, self.input.buf, ot=bl.f32)
+
+    def transfer_in(self, buf):
+        self.copy(self.input.buf, buf)
+
+    def transfer_out(self, buf):
+        self.copy(self.output.buf, self.module.detector.get_output())
+        self.copy(buf, self.output.buf)
+
+
+@pytest.mark.parametrize("npol", [1, 4])
+def test_detector_module(npol):
+    input_shape = (2, 192, 8750, 2)
+    intg = 10
+    output_shape = (2, 192, 8750 // intg, npol)
+    rng = np.random.default_rng(npol)
+    host_input = bl.array_tensor(input_shape, dtype=bl.cf32, device=bl.cpu)
+    host_output = bl.array_tensor(output_shape, dtype=bl.f32, device=bl.cpu)
+    fill(host_input, crand(rng, input_shape))
+    pipeline = DetectorPipeline(input_shape, output_shape,
+                                {"integration_size": intg, "number_of_output_polarizations": npol})
+    pipeline(host_input, host_output)
+    ref = O.detector(host_input.as_numpy(), intg, npol)
+    assert O.norm_rel_err(host_output.as_numpy(), ref) <= TOL
+
+
+def test_detector_module_stepped_integration():
+    # integration over 2 blocks: CHRONOUS, output only every second call
+    input_shape = (2, 16, 64, 2)
+    output_shape = (2, 16, 1, 1)
+    rng = np.random.default_rng(9)
+    host_input = bl.array_tensor(input_shape, dtype=bl.cf32, device=bl.cpu)
+    host_output = bl.array_tensor(output_shape, dtype=bl.f32, device=bl.cpu)
+    pipeline = DetectorPipeline(input_shape, output_shape,
+                                {"integration_size": 128, "number_of_output_polarizations": 1})
+    blocks = [crand(rng, input_shape) for _ in range(2)]
+    fill(host_input, blocks[0])
+    assert pipeline(host_input, host_output) is False
+    fill(host_input, blocks[1])
+    assert pipeline(host_input, host_output) is True
+    ref = O.detector_stepped(blocks, 128, 1)
+    assert O.norm_rel_err(host_output.as_numpy(), ref) <= TOL
+
+
+# ------------------------------------------------------------------ cast.py
+
+@bl.runner
+class CastPipeline:
+    def __init__(self, shape):
+        self.input.buf = bl.array_tensor(shape, dtype=bl.i8)
+        self.output.buf = bl.array_tensor(shape, dtype=bl.f32)
+        self.module.cast = bl.module(bl.cast, {}, self.input.buf, it=bl.i8, ot=bl.f32)
+
+    def transfer_in(self, buf):
+        self.copy(self.input.buf, buf)
+
+    def transfer_out(self, buf):
+        self.copy(self.output.buf, self.module.cast.get_output())
+        self.copy(buf, self.output.buf)
+
+
+def test_cast_module():
+    shape = (2, 192, 8750, 2)
+    rng = np.random.default_rng(3)
+    host_input = bl.array_tensor(shape, dtype=bl.i8, device=bl.cpu)
+    host_output = bl.array_tensor(shape, dtype=bl.f32, device=bl.cpu)
+    fill(host_input, rng.integers(-128, 128, shape).astype(np.int8))
+    CastPipeline(shape)(host_input, host_output)
+    assert np.array_equal(host_output.as_numpy(), host_input.as_numpy().astype(np.float32))
+
+
+# ---------------------------------------------------------------- gather.py
+
+@bl.runner
+class GatherPipeline:
+    def __init__(self, in_shape, out_shape, config):
+        self.input.buf = bl.array_tensor(in_shape, dtype=bl.cf32)
+        self.output.buf = bl.array_tensor(out_shape, dtype=bl.cf32)
+        self.module.gather = bl.module(bl.gather, config, self.input.buf)
+
+    def transfer_in(self, buf):
+        self.copy(self.input.buf, buf)
+
+    def transfer_out(self, buf):
+        self.copy(self.output.buf, self.module.gather.get_output())
+        self.copy(buf, self.output.buf)
+
+
+@pytest.mark.parametrize("A,F,T,P,axis,mult", [(2, 192, 8192, 2, 2, 16), (3, 7, 3, 1, 0, 5),
+                                               (1, 2, 5, 6, 1, 6), (11, 1, 13, 9, 2, 7),
+                                               (9, 2, 1, 3, 3, 8), (9, 2, 5555, 3, 2, 8)])
+def test_gather_module(A, F, T, P, axis, mult):
+    in_shape = (A, F, T, P)
+    out_shape = list(in_shape)
+    out_shape[axis] *= mult
+    rng = np.random.default_rng(A * T)
+    host_input = bl.array_tensor(in_shape, dtype=bl.cf32, device=bl.cpu)
+    host_output = bl.array_tensor(tuple(out_shape), dtype=bl.cf32, device=bl.cpu)
+    pipeline = GatherPipeline(in_shape, tuple(out_shape), {"axis": axis, "multiplier": mult})
+    blocks = []
+    done = False
+    while not done:
+        blk = crand(rng, in_shape)
+        blocks.append(blk)
+        fill(host_input, blk)
+        done = pipeline(host_input, host_output)
+    assert len(blocks) == mult
+    assert np.array_equal(host_output.as_numpy(), O.gather(blocks, axis))
+
+
+# ------------------------------------------------------------ permutation.py
+
+@bl.runner
+class PermutationPipeline:
+    def __init__(self, in_shape, out_shape, config):
+        self.input.buf = bl.array_tensor(in_shape, dtype=bl.cf32)
+        self.output.buf = bl.array_tensor(out_shape, dtype=bl.cf32)
+        self.module.permutation = bl.module(bl.permutation, config, self.input.buf)
+
+    def transfer_in(self, buf):
+        self.copy(self.input.buf, buf)
+
+    def transfer_out(self, buf):
+        self.copy(self.output.buf, self.module.permutation.get_output())
+        self.copy(buf, self.output.buf)
+
+
+@pytest.mark.parametrize("idx", [[0, 1, 2, 3], [0, 2, 1, 3], [3, 2, 1, 0], [2, 0, 3, 1]])
+def test_permutation_module(idx):
+    in_shape = (2, 192, 8192, 2)
+    out_shape = tuple(in_shape[i] for i in idx)
+    rng = np.random.default_rng(sum(idx))
+    host_input = bl.array_tensor(in_shape, dtype=bl.cf32, device=bl.cpu)
+    host_output = bl.array_tensor(out_shape, dtype=bl.cf32, device=bl.cpu)
+    fill(host_input, crand(rng, in_shape))
+    PermutationPipeline(in_shape, out_shape, {"indexes": idx})(host_input, host_output)
+    assert np.array_equal(host_output.as_numpy(), np.transpose(host_input.as_numpy(), idx))
+
+
+# ------------------------------------------------------------ bundles (mode_b)
+
+ATA_POS = [(-2524041.5388905862, -4123587.965024342, 4147646.4222955606),
+           (-2524068.187873109, -4123558.735413135, 4147656.21282186),
+           (-2524087.2078100787, -4123532.397416349, 4147670.9866770394),
+           (-2524103.384010733, -4123511.111598937, 4147682.4133068994),
+           (-2524056.730228759, -4123515.287949227, 4147706.4850287656),
+           (-2523986.279601761, -4123497.427940991, 4147766.732988923)]
+
+
+def modeb_config(in_shape, out_shape, cal, rate, circ, det):
+    return {
+        "input_shape": in_shape, "output_shape": out_shape,
+        "pre_beamformer_channelizer_rate": rate,
+        "pre_beamformer_polarizer_convert_to_circular": circ,
+        "phasor_observation_frequency_hz": 6500.125e6, "phasor_channel_bandwidth_hz": 0.5e6,
+        "phasor_total_bandwidth_hz": 1.024e9, "phasor_frequency_start_index": 352,
+        "phasor_reference_antenna_index": 0,
+        "phasor_array_reference_position": bl.lla(lon=math.radians(-121.470733),
+                                                  lat=math.radians(40.815987), alt=1020.86),
+        "phasor_boresight_coordinate": bl.ra_dec(ra=0.64169, dec=1.079896295),
+        "phasor_antenna_positions": [bl.xyz(*p) for p in ATA_POS[:in_shape[0]]],
+        "phasor_antenna_calibrations": cal,
+        "phasor_beam_coordinates": [bl.ra_dec(ra=0.63722, dec=1.07552424),
+                                    bl.ra_dec(ra=0.65063, dec=1.08426835)],
+        "beamformer_incoherent_beam": True,
+        "detector_enable": det, "detector_integration_size": 1,
+        "detector_number_of_output_polarizations": 1,
+    }
+
+
+@pytest.mark.parametrize("it,rate,circ", [("ci8", 1, False), ("cf32", 1, False), ("cf32", 4, True)])
+def test_mode_b_bundle(it, rate, circ):
+    # tests/bundles/ata/mode_b.py, with the module's own delays feeding the
+    # oracle phasors (the delay engine itself is checked separately).
+    A, F, T = 6, 16, 512
+    in_shape = (A, F, T, 2)
+    out_shape = (3, F * rate, T // rate, 1)
+    rng = np.random.default_rng(5)
+    cal = bl.array_tensor((A, F * rate, 1, 2), dtype=bl.cf64, device=bl.cpu)
+    fill(cal, rng.random((A, F * rate, 1, 2)) + 1j * rng.random((A, F * rate, 1, 2)))
+    dtype = bl.ci8 if it == "ci8" else bl.cf32
+    x8 = O.ci8_gaussian(in_shape, rng=rng)
+
+    @bl.runner
+    class Pipeline:
+        def __init__(self):
+            self.input.dut = bl.tensor(1, dtype=bl.f64, device=bl.cpu)
+            self.input.date = bl.tensor(1, dtype=bl.f64, device=bl.cpu)
+            self.input.buffer = bl.array_tensor(in_shape, dtype=dtype)
+            self.output.buffer = bl.array_tensor(out_shape, dtype=bl.f32)
+            self.module.mode_b = bl.module(bl.modeb, modeb_config(in_shape, out_shape, cal, rate, circ, True),
+                                           (self.input.dut, self.input.date, self.input.buffer),
+                                           it=dtype, ot=bl.f32, telescope=bl.ata)
+
+        def transfer_in(self, dut, date, buffer):
+            self.copy(self.input.dut, dut)
+            self.copy(self.input.date, date)
+            self.copy(self.input.buffer, buffer)
+
+        def transfer_out(self, buffer):
+            self.copy(self.output.buffer, self.module.mode_b.get_output())
+            self.copy(buffer, self.output.buffer)
+
+    host_dut = bl.tensor(1, dtype=bl.f64, device=bl.cpu)
+    host_date = bl.tensor(1, dtype=bl.f64, device=bl.cpu)
+    host_in = bl.array_tensor(in_shape, dtype=dtype, device=bl.cpu)
+    host_out = bl.array_tensor(out_shape, dtype=bl.f32, device=bl.cpu)
+    host_date.as_numpy()[0] = (1649366473.0 / 86400) + 2440587.5
+    host_dut.as_numpy()[0] = 0.0
+    fill(host_in, x8 if it == "ci8" else O.ci8_to_complex(x8).astype(np.complex64))
+    p = Pipeline()
+    p(host_dut, host_date, host_in, host_out)
+    # phasors of the module's delays, then the oracle chain
+    phasor_mod = [m for m in p.runner._keep if m.name() == "Phasor"][0] if hasattr(p.runner, "_keep") else None
+    delays = phasor_mod.get_delays() if phasor_mod is not None else None
+    hd = bl.delay_tensor((2, A), dtype=bl.f64, device=bl.cpu)
+    bl.copy(hd, delays)
+    w = O.phasor_ata(hd.as_numpy(), cal.as_numpy(), 6500.125e6, 0.5e6, 1.024e9, 352)
+    ref = O.mode_b(x8, w, channelizer_rate=rate, convert_to_circular=circ, incoherent_beam=True,
+                   detector_enable=True, detector_integration_size=1,
+                   detector_number_of_output_polarizations=1)
+    assert O.norm_rel_err(host_out.as_numpy(), ref) <= 2e-5
+
+
+def test_phasor_delays_geometry():
+    # Delays of the built-in engine: zero for the reference antenna, bounded by
+    # baseline/c, and beam == boresight gives zero (tests/modules/phasor/advanced.py
+    # observation constants).
+    A, F = 6, 8
+    cal = bl.array_tensor((A, F, 1, 2), dtype=bl.cf64, device=bl.cpu)
+    fill(cal, np.ones((A, F, 1, 2)))
+    date = bl.tensor(1, dtype=bl.f64, device=bl.cpu)
+    dut = bl.tensor(1, dtype=bl.f64, device=bl.cpu)
+    date.as_numpy()[0] = (1649366473.0 / 86400) + 2440587.5
+    dut.as_numpy()[0] = 0.0
+    cfg = {
+        "number_of_antennas": A, "number_of_frequency_channels": F, "number_of_polarizations": 2,
+        "observation_frequency_hz": 6500.125e6, "channel_bandwidth_hz": 0.5e6,
+        "total_bandwidth_hz": 1.024e9, "frequency_start_index": 352, "reference_antenna_index": 0,
+        "array_reference_position": bl.lla(lon=math.radians(-121.470733), lat=math.radians(40.815987),
+                                           alt=1020.86),
+        "boresight_coordinate": bl.ra_dec(ra=0.64169, dec=1.079896295),
+        "antenna_positions": [bl.xyz(*p) for p in ATA_POS[:A]],
+        "antenna_calibrations": cal,
+        "beam_coordinates": [bl.ra_dec(ra=0.64169, dec=1.079896295), bl.ra_dec(ra=0.65063, dec=1.08426835)],
+    }
+    mod = bl.module(bl.phasor, cfg, (date, dut), telescope=bl.ata)
+    assert mod.process(0) == bl.result.success
+    d = mod.get_delays().as_numpy()
+    assert np.all(d[0] == 0.0)                 # beam at the boresight
+    assert d[1, 0] == 0.0                      # reference antenna
+    base = np.linalg.norm(np.array(ATA_POS[:A]) - np.array(ATA_POS[0]), axis=1)
+    assert np.all(np.abs(d[1]) <= base / 299792458.0 * 0.05 + 1e-15)   # small offset beam
+    ph = bl.phasor_tensor((2, A, F, 1, 2), dtype=bl.cf32, device=bl.cpu)
+    bl.copy(ph, mod.get_phasors())
+    ref = O.phasor_ata(d, np.ones((A, F, 1, 2)), 6500.125e6, 0.5e6, 1.024e9, 352)
+    assert np.max(np.abs(ph.as_numpy() - ref)) < 2e-7

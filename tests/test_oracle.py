@@ -284,3 +284,16 @@ def test_calibration_gradient_closed_form():
         dre = (O.beamformed_stokes_i_loss(x, g + e, w) - O.beamformed_stokes_i_loss(x, g - e, w)) / (2 * h)
         dim = (O.beamformed_stokes_i_loss(x, g + 1j * e, w) - O.beamformed_stokes_i_loss(x, g - 1j * e, w)) / (2 * h)
         assert np.isclose(grad[idx].real, dre, rtol=1e-6) and np.isclose(grad[idx].imag, dim, rtol=1e-6)
+
+
+def test_cpu_port_matches_oracle():
+    x8 = O.ci8_gaussian((5, 3, 32, 2), rng=3)
+    w = crand(4, 5, 3, 1, 2)
+    for rate, pol in ((1, False), (4, True), (2, False)):
+        ref = O.beamformer_ata(x8, w)
+        ref = O.channelizer(ref, rate)
+        if pol:
+            ref = O.polarizer(ref)
+        got = O.beamform_chain_c64(x8, w, rate, pol)
+        assert got.dtype == np.complex64
+        assert O.norm_rel_err(got, ref) < 1e-6
