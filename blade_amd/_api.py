@@ -123,11 +123,15 @@ def _resolve(t, slot):
     return t[slot] if _is_duet(t) else t
 
 
+def _dims(shape):
+    return tuple(shape[i] for i in range(len(shape)))
+
+
 def _copy(dst, src, stream_handle=0, sync=True):
     if dst.size_bytes() != src.size_bytes():
         raise ValueError(f"Size mismatch between source ({src.size_bytes()} B) and destination "
                          f"({dst.size_bytes()} B).")
-    if list(dst.shape) != list(src.shape):
+    if _dims(dst.shape) != _dims(src.shape):
         raise ValueError(f"Shape mismatch between source ({src.shape}) and destination ({dst.shape}).")
     return _ext._copy_impl.copy_bytes(dst.data_ptr(), src.data_ptr(), dst.size_bytes(),
                                       stream_handle, sync)

@@ -32,15 +32,20 @@ SIGNATURES = {
     "blade_cast": [_i, _i, _p, _p, _u64, _p],
     "blade_beamformer_ata": [_i, _i, _p, _p, _p, _u64, _u64, _u64, _u64, _u64, _i, _i, _p],
     "blade_beamform_chain": [_i, _i, _p, _p, _p, _u64, _u64, _u64, _u64, _u64, _i, _i, _u64, _i, _p],
+    "blade_beamformer_workspace_size": [_i, _u64, _u64, _u64],
+    "blade_beamform_chain_ws": [_i, _i, _p, _p, _p, _u64, _u64, _u64, _u64, _u64, _i, _i, _u64, _i,
+                                _p, _u64, _p],
     "blade_channelizer": [_p, _p, _u64, _u64, _u64, _u64, _u64, _p],
     "blade_channelizer_plan": [_u64, _p],
+    "blade_channelizer_detector": [_p, _p, _u64, _u64, _u64, _u64, _u64, _u64, _i, _p],
     "blade_polarizer": [_i, _p, _p, _u64, _p],
     "blade_detector": [_p, _p, _u64, _u64, _u64, _u64, _u64, _i, _p],
     "blade_gather": [_p, _p, _u64, _u64, _u64, _u64, _p],
     "blade_permutation": [_p, _p, _u64, ct.POINTER(_u64), ct.POINTER(_u64), _p],
     "blade_phasor_ata": [_p, _p, _p, _i, _u64, _u64, _u64, _u64, _d, _d, _d, _u64, _p],
 }
-_RESTYPES = {"blade_amd_version": ct.c_char_p, "blade_amd_last_error": ct.c_char_p}
+_RESTYPES = {"blade_amd_version": ct.c_char_p, "blade_amd_last_error": ct.c_char_p,
+             "blade_beamformer_workspace_size": ct.c_uint64}
 
 _lib = None
 

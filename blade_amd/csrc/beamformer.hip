@@ -35,6 +35,8 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <map>
+#include <mutex>
 
 namespace bl {
 
@@ -50,16 +52,18 @@ __device__ __forceinline__ floatx4 mfma_f16(halfx8 a, halfx8 b, floatx4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
-// Exact CI8 -> FP16 for one pol of a dword (re0, im0, re1, im1): the biased
-// byte u = b + 128 is placed in the mantissa of 1024.0 (0x6400), giving
-// 1024 + u exactly, and 1152 is subtracted.
-__device__ __forceinline__ halfx2 ci8_pol_to_half2(uint32_t d, int p) {
+// Exact CI8 -> FP16: int8 b becomes the biased byte u = b + 128, placed in the
+// mantissa of 1024.0 (0x6400) to give 1024 + u exactly; subtracting 1152 gives b.
+// Both pols of a dword at once: v_perm_b32 places each biased byte below a
+// 0x64 byte (0x64uu = 1024 + u in FP16), one packed subtract removes the bias.
+// {src0, src1} is read as bytes 7..4 = src0, 3..0 = src1.
+__device__ __forceinline__ void ci8_to_half2x2(uint32_t d, halfx2& p0, halfx2& p1) {
     const uint32_t u = d ^ 0x80808080u;
-    const uint32_t sh = 16u * p;
-    const uint32_t bits = ((u >> sh) & 0xFFu) | (((u >> (sh + 8)) & 0xFFu) << 16) | 0x64006400u;
-    halfx2 h = __builtin_bit_cast(halfx2, bits);
+    const uint32_t b0 = __builtin_amdgcn_perm(u, 0x64646464u, 0x00050004u);   // (u0, 64, u1, 64)
+    const uint32_t b1 = __builtin_amdgcn_perm(u, 0x64646464u, 0x00070006u);   // (u2, 64, u3, 64)
     const halfx2 bias = {(_Float16)1152.0f, (_Float16)1152.0f};
-    return h - bias;
+    p0 = __builtin_bit_cast(halfx2, b0) - bias;
+    p1 = __builtin_bit_cast(halfx2, b1) - bias;
 }
 
 __device__ __forceinline__ float ci8_byte(uint32_t d, int idx) {
@@ -77,7 +81,117 @@ struct BfArgs {
     int nsteps;             // K steps (FP32: 2 per antenna group; FP16: 1 per 16 antennas)
     int n_tchunks;          // chunks of WAVES*TILE*NT samples along T
     int chunks_per_wg;      // consecutive chunks walked by one workgroup
+    unsigned char* pack;    // packed phasor slabs, one per (f, beam group)
+    size_t slab_bytes;      // bytes per slab
+    size_t wbytes;          // bytes of A operands at the head of a slab
 };
+
+// Packs the phasors of one (f, beam group) into the A-operand layout the
+// beamformer copies verbatim into LDS, followed by |w0|^2[p][a] (256 floats)
+// and the split-FP16 unscale factors (2 floats).
+//   FP32       [m][p][step][g][row] float, step = 2 * antenna group + component
+//   split FP16 [m][p][step][hl][lane] halfx8 (8 k values = 4 antennas x re/im),
+//              components scaled by a per-pol power of two so that all are < 1,
+//              hi = fp16(w), lo = fp16(w - hi).
+template <int MW, bool F16X2>
+__global__ __launch_bounds__(256) void bf_pack_kernel(BfArgs a) {
+    constexpr int AK = F16X2 ? 4 * KG : KG;
+    __shared__ unsigned int s_max[2];
+    const int f = blockIdx.y;
+    const int b_wg = blockIdx.z * MW * (TILE / 2);
+    const int nsteps = a.nsteps;
+    unsigned char* slab = a.pack + ((size_t)f * gridDim.z + blockIdx.z) * a.slab_bytes;
+    float* tail = reinterpret_cast<float*>(slab + a.wbytes);
+    if (threadIdx.x < 2) s_max[threadIdx.x] = 0u;
+    __syncthreads();
+    float unscale[2] = {1.f, 1.f};
+    if constexpr (F16X2) {
+        float mx[2] = {0.f, 0.f};
+        for (int e = threadIdx.x; e < MW * (TILE / 2) * a.A; e += blockDim.x) {
+            const int bl_ = e / a.A, an = e - bl_ * a.A, b = b_wg + bl_;
+            if (b < a.B) {
+                const float4 w = reinterpret_cast<const float4*>(a.w)[((size_t)b * a.A + an) * a.F + f];
+                mx[0] = fmaxf(mx[0], fmaxf(fabsf(w.x), fabsf(w.y)));
+                mx[1] = fmaxf(mx[1], fmaxf(fabsf(w.z), fabsf(w.w)));
+            }
+        }
+        atomicMax(&s_max[0], __float_as_uint(mx[0]));
+        atomicMax(&s_max[1], __float_as_uint(mx[1]));
+        __syncthreads();
+        float scale[2];
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            int ex = 0;
+            frexpf(__uint_as_float(s_max[p]), &ex);
+            scale[p] = ldexpf(1.0f, -ex);
+            unscale[p] = ldexpf(1.0f, ex);
+        }
+        halfx8* wh = reinterpret_cast<halfx8*>(slab);
+        const int total = MW * 2 * nsteps * 2 * 64;
+        for (int e = threadIdx.x; e < total; e += blockDim.x) {
+            const int ln = e & 63;
+            const int hl = (e >> 6) & 1;
+            const int r = e >> 7;                 // (m * 2 + p) * nsteps + st
+            const int st = r % nsteps;
+            const int p = (r / nsteps) & 1;
+            const int m = (r / nsteps) >> 1;
+            const int row = ln % TILE, kg = ln / TILE;
+            const int b = b_wg + m * (TILE / 2) + row / 2, co = row & 1;
+            halfx8 h;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int an = st * AK + 4 * kg + q;
+                float wr = 0.f, wi = 0.f;
+                if (b < a.B && an < a.A) {
+                    const float2 w = a.w[(((size_t)b * a.A + an) * a.F + f) * 2 + p];
+                    wr = w.x * scale[p];
+                    wi = w.y * scale[p];
+                }
+                // k = (antenna, c): c = 0 multiplies Re x, c = 1 multiplies Im x.
+                const float v0 = co == 0 ? wr : wi;
+                const float v1 = co == 0 ? -wi : wr;
+                const _Float16 h0 = (_Float16)v0, h1 = (_Float16)v1;
+                if (hl == 0) {
+                    h[2 * q] = h0;
+                    h[2 * q + 1] = h1;
+                } else {
+                    h[2 * q] = (_Float16)(v0 - (float)h0);
+                    h[2 * q + 1] = (_Float16)(v1 - (float)h1);
+                }
+            }
+            wh[e] = h;
+        }
+    } else {
+        float* wl = reinterpret_cast<float*>(slab);
+        const int total = MW * 2 * nsteps * 64;
+        for (int e = threadIdx.x; e < total; e += blockDim.x) {
+            const int row = e % TILE;
+            const int kg = (e / TILE) % KG;
+            const int r = e / 64;                 // (m * 2 + p) * nsteps + st
+            const int st = r % nsteps;
+            const int p = (r / nsteps) & 1;
+            const int m = (r / nsteps) >> 1;
+            const int an = (st / 2) * KG + kg, c = st & 1;
+            const int b = b_wg + m * (TILE / 2) + row / 2, co = row & 1;
+            float v = 0.f;
+            if (b < a.B && an < a.A) {
+                const float2 w = a.w[(((size_t)b * a.A + an) * a.F + f) * 2 + p];
+                v = (co == 0) ? (c == 0 ? w.x : -w.y) : (c == 0 ? w.y : w.x);
+            }
+            wl[e] = v;
+        }
+    }
+    for (int e = threadIdx.x; e < 2 * 128; e += blockDim.x) {
+        const int p = e / 128, an = e % 128;
+        float v = 0.f;
+        if (an < a.A) {
+            const float2 w = a.w[((size_t)an * a.F + f) * 2 + p];   // beam 0
+            v = w.x * w.x + w.y * w.y;
+        }
+        tail[e] = v;
+    }
+    if (threadIdx.x < 2) tail[256 + threadIdx.x] = unscale[threadIdx.x];
+}
 
 __device__ __forceinline__ void polarize2(float2& X, float2& Y) {
     const float2 L = make_float2(X.x - Y.y, X.y + Y.x);
@@ -184,7 +298,7 @@ __global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
     typedef Unit<NT, CI8, F16X2> U_t;
 
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    __shared__ unsigned int s_max[2];
+    __shared__ float s_unscale[2];
     __shared__ float s_w0sq[2][128];
 
     const int lane = threadIdx.x & 63;
@@ -238,103 +352,23 @@ __global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
 #pragma unroll
     for (int d = 0; d < D; d++) load_unit(d, ring[d]);
 
-    // ---- Stage phasors for this (f, beam group) into LDS. -------------------
-    if (threadIdx.x < 2) s_max[threadIdx.x] = 0u;
-    __syncthreads();
-    if constexpr (F16X2) {
-        // Per-pol power-of-two scale so that every scaled component is < 1.
-        float mx[2] = {0.f, 0.f};
-        for (int e = threadIdx.x; e < MW * (TILE / 2) * a.A; e += blockDim.x) {
-            const int bl_ = e / a.A, an = e - bl_ * a.A, b = b_wg + bl_;
-            if (b < a.B) {
-                const float4 w = reinterpret_cast<const float4*>(a.w)[((size_t)b * a.A + an) * a.F + f];
-                mx[0] = fmaxf(mx[0], fmaxf(fabsf(w.x), fabsf(w.y)));
-                mx[1] = fmaxf(mx[1], fmaxf(fabsf(w.z), fabsf(w.w)));
-            }
+    // ---- Copy this (f, beam group)'s packed phasors into LDS. ---------------
+    // The slab was laid out by bf_pack_kernel: A operands, |w0|^2 per
+    // antenna and pol, and the power-of-two unscale of the split-FP16 path.
+    const unsigned char* slab = a.pack + ((size_t)f * gridDim.z + blockIdx.z) * a.slab_bytes;
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(slab);
+        uint4* dst = reinterpret_cast<uint4*>(lds_raw);
+        const int n16 = (int)(a.wbytes / 16);
+        for (int e = threadIdx.x; e < n16; e += blockDim.x) dst[e] = src[e];
+        const float* tail = reinterpret_cast<const float*>(slab + a.wbytes);
+        if (do_inc) {
+            for (int e = threadIdx.x; e < 2 * 128; e += blockDim.x) (&s_w0sq[0][0])[e] = tail[e];
         }
-        atomicMax(&s_max[0], __float_as_uint(mx[0]));
-        atomicMax(&s_max[1], __float_as_uint(mx[1]));
-        __syncthreads();
-        halfx8* wh = reinterpret_cast<halfx8*>(lds_raw);
-        float scale[2];
-#pragma unroll
-        for (int p = 0; p < 2; p++) {
-            int ex = 0;
-            frexpf(__uint_as_float(s_max[p]), &ex);
-            scale[p] = ldexpf(1.0f, -ex);
-        }
-        const int total = MW * 2 * nsteps * 2 * 64;
-        for (int e = threadIdx.x; e < total; e += blockDim.x) {
-            const int ln = e & 63;
-            const int hl = (e >> 6) & 1;
-            const int r = e >> 7;                 // (m * 2 + p) * nsteps + st
-            const int st = r % nsteps;
-            const int p = (r / nsteps) & 1;
-            const int m = (r / nsteps) >> 1;
-            const int row = ln % TILE, kg = ln / TILE;
-            const int b = b_wg + m * (TILE / 2) + row / 2, co = row & 1;
-            halfx8 h;
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int an = st * AK + 4 * kg + q;
-                float wr = 0.f, wi = 0.f;
-                if (b < a.B && an < a.A) {
-                    const float2 w = a.w[(((size_t)b * a.A + an) * a.F + f) * 2 + p];
-                    wr = w.x * scale[p];
-                    wi = w.y * scale[p];
-                }
-                // k = (antenna, c): c = 0 multiplies Re x, c = 1 multiplies Im x.
-                const float v0 = co == 0 ? wr : wi;
-                const float v1 = co == 0 ? -wi : wr;
-                const _Float16 h0 = (_Float16)v0, h1 = (_Float16)v1;
-                if (hl == 0) {
-                    h[2 * q] = h0;
-                    h[2 * q + 1] = h1;
-                } else {
-                    h[2 * q] = (_Float16)(v0 - (float)h0);
-                    h[2 * q + 1] = (_Float16)(v1 - (float)h1);
-                }
-            }
-            wh[e] = h;
-        }
-    } else {
-        float* wl = reinterpret_cast<float*>(lds_raw);
-        const int total = MW * 2 * nsteps * 64;
-        for (int e = threadIdx.x; e < total; e += blockDim.x) {
-            const int row = e % TILE;
-            const int kg = (e / TILE) % KG;
-            const int r = e / 64;                 // (m * 2 + p) * nsteps + st
-            const int st = r % nsteps;
-            const int p = (r / nsteps) & 1;
-            const int m = (r / nsteps) >> 1;
-            const int an = (st / 2) * KG + kg, c = st & 1;
-            const int b = b_wg + m * (TILE / 2) + row / 2, co = row & 1;
-            float v = 0.f;
-            if (b < a.B && an < a.A) {
-                const float2 w = a.w[(((size_t)b * a.A + an) * a.F + f) * 2 + p];
-                v = (co == 0) ? (c == 0 ? w.x : -w.y) : (c == 0 ? w.y : w.x);
-            }
-            wl[e] = v;
-        }
-    }
-    if (do_inc) {
-        for (int e = threadIdx.x; e < 2 * a.A; e += blockDim.x) {
-            const int an = e >> 1, p = e & 1;
-            const float2 w = a.w[((size_t)an * a.F + f) * 2 + p];   // beam 0
-            if (an < 128) s_w0sq[p][an] = w.x * w.x + w.y * w.y;
-        }
+        if (threadIdx.x < 2) s_unscale[threadIdx.x] = tail[256 + threadIdx.x];
     }
     __syncthreads();
-
-    float unscale[2] = {1.f, 1.f};
-    if constexpr (F16X2) {
-#pragma unroll
-        for (int p = 0; p < 2; p++) {
-            int ex = 0;
-            frexpf(__uint_as_float(s_max[p]), &ex);
-            unscale[p] = ldexpf(1.0f, ex);
-        }
-    }
+    const float unscale[2] = {s_unscale[0], s_unscale[1]};
 
     floatx4 acc[MW][2][NT];
     float inc[2][NT];
@@ -358,29 +392,33 @@ __global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
     auto compute = [&](const U_t& u, int un) {
         if constexpr (F16X2) {
             const halfx8* wh = reinterpret_cast<const halfx8*>(lds_raw);
+            halfx8 whi[2][MW], wlo[2][MW];
 #pragma unroll
-            for (int p = 0; p < 2; p++) {
-                halfx8 whi[MW], wlo[MW];
+            for (int p = 0; p < 2; p++)
 #pragma unroll
                 for (int m = 0; m < MW; m++) {
-                    whi[m] = wh[(((m * 2 + p) * nsteps + un) * 2 + 0) * 64 + lane];
-                    wlo[m] = wh[(((m * 2 + p) * nsteps + un) * 2 + 1) * 64 + lane];
+                    whi[p][m] = wh[(((m * 2 + p) * nsteps + un) * 2 + 0) * 64 + lane];
+                    wlo[p][m] = wh[(((m * 2 + p) * nsteps + un) * 2 + 1) * 64 + lane];
                 }
 #pragma unroll
-                for (int n = 0; n < NT; n++) {
-                    halfx8 xb;
+            for (int n = 0; n < NT; n++) {
+                halfx8 xb[2];   // B operands of both pols from the same four dwords
 #pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        const halfx2 h = ci8_pol_to_half2(u.w[q * NT + n], p);
-                        xb[2 * q] = h[0];
-                        xb[2 * q + 1] = h[1];
-                    }
+                for (int q = 0; q < 4; q++) {
+                    halfx2 h0, h1;
+                    ci8_to_half2x2(u.w[q * NT + n], h0, h1);
+                    xb[0][2 * q] = h0[0];
+                    xb[0][2 * q + 1] = h0[1];
+                    xb[1][2 * q] = h1[0];
+                    xb[1][2 * q + 1] = h1[1];
+                }
+#pragma unroll
+                for (int p = 0; p < 2; p++)
 #pragma unroll
                     for (int m = 0; m < MW; m++) {
-                        acc[m][p][n] = mfma_f16(whi[m], xb, acc[m][p][n]);
-                        acc[m][p][n] = mfma_f16(wlo[m], xb, acc[m][p][n]);
+                        acc[m][p][n] = mfma_f16(whi[p][m], xb[p], acc[m][p][n]);
+                        acc[m][p][n] = mfma_f16(wlo[p][m], xb[p], acc[m][p][n]);
                     }
-                }
             }
             if (do_inc) {
 #pragma unroll
@@ -510,15 +548,65 @@ static int env_int(const char* name, int dflt) {
     return v ? atoi(v) : dflt;
 }
 
+// Bytes of one packed phasor slab and of its A-operand head.
+static void slab_sizes(bool f16x2, int mw, int nsteps, size_t* wbytes, size_t* slab) {
+    *wbytes = f16x2 ? (size_t)mw * 2 * nsteps * 2 * 64 * 16 : (size_t)mw * 2 * nsteps * 64 * 4;
+    *slab = ((*wbytes + 258 * sizeof(float)) + 255) / 256 * 256;
+}
+
+static int nsteps_for(bool f16x2, int A) {
+    return f16x2 ? (A + 4 * KG - 1) / (4 * KG) : 2 * ((A + KG - 1) / KG);
+}
+
+static int groups_for(int B, int mw) { return (B + mw * (TILE / 2) - 1) / (mw * (TILE / 2)); }
+
+// Library-owned pack buffers, one per (device, stream) so that calls on
+// different streams never share one; grown on demand.
+static std::mutex g_ws_mutex;
+static std::map<std::pair<int, hipStream_t>, std::pair<unsigned char*, size_t>> g_ws;
+
+static int library_workspace(size_t bytes, hipStream_t s, unsigned char** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail("beamformer: hipGetDevice failed");
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    auto& e = g_ws[{dev, s}];
+    if (e.second < bytes) {
+        if (e.first) {
+            // The old buffer may still be read by work queued on this stream.
+            if (hipStreamSynchronize(s) != hipSuccess || hipFree(e.first) != hipSuccess) {
+                set_error("beamformer: could not release the old workspace");
+                return BLADE_HIP_ERROR;
+            }
+            e = {nullptr, 0};
+        }
+        void* p = nullptr;
+        hipError_t err = hipMalloc(&p, bytes);
+        if (err != hipSuccess) {
+            set_error(std::string("beamformer: workspace allocation: ") + hipGetErrorString(err));
+            return BLADE_HIP_ERROR;
+        }
+        e = {static_cast<unsigned char*>(p), bytes};
+    }
+    *out = e.first;
+    return BLADE_SUCCESS;
+}
+
 template <int NT, int MW, bool CI8, bool F16X2, int D>
-static int launch_bf(const BfArgs& args, hipStream_t s) {
+static int launch_bf(const BfArgs& args, unsigned char* ws, size_t ws_bytes, hipStream_t s) {
     BfArgs a = args;
-    if (F16X2) a.nsteps = (a.A + 4 * KG - 1) / (4 * KG);
-    else a.nsteps = 2 * ((a.A + KG - 1) / KG);
-    const size_t lds = F16X2 ? (size_t)MW * 2 * a.nsteps * 2 * 64 * 16
-                             : (size_t)MW * 2 * a.nsteps * 64 * 4;
+    a.nsteps = nsteps_for(F16X2, a.A);
+    slab_sizes(F16X2, MW, a.nsteps, &a.wbytes, &a.slab_bytes);
+    const size_t lds = a.wbytes;
     const int span = WAVES * TILE * NT;
-    const int groups = (a.B + MW * (TILE / 2) - 1) / (MW * (TILE / 2));
+    const int groups = groups_for(a.B, MW);
+    const size_t need = a.slab_bytes * (size_t)a.F * groups;
+    if (ws) {
+        if (ws_bytes < need) return fail("beamformer: workspace too small");
+        a.pack = ws;
+    } else {
+        const int rc = library_workspace(need, s, &a.pack);
+        if (rc) return rc;
+    }
     a.n_tchunks = (a.T + span - 1) / span;
     // Chunks per workgroup: enough workgroups for ~3 rounds of 2 per CU.
     int cpw = env_int("BLADE_BF_CHUNKS_PER_WG", 0);
@@ -539,30 +627,31 @@ static int launch_bf(const BfArgs& args, hipStream_t s) {
             return BLADE_HIP_ERROR;
         }
     }
+    hipLaunchKernelGGL((bf_pack_kernel<MW, F16X2>), dim3(1, (unsigned)a.F, (unsigned)groups), dim3(256), 0, s, a);
     hipLaunchKernelGGL((beamform_kernel<NT, MW, CI8, F16X2, D>), grid, dim3(64 * WAVES), lds, s, a);
     return check_launch("beamformer");
 }
 
 // Variant table: (time samples per lane, beam tiles per wave) by precision.
 template <bool CI8, bool F16X2>
-static int dispatch(const BfArgs& a, int nt, int mw, hipStream_t s) {
+static int dispatch(const BfArgs& a, int nt, int mw, unsigned char* ws, size_t wsb, hipStream_t s) {
     if constexpr (F16X2) {
-        if (nt == 4) return mw == 2 ? launch_bf<4, 2, true, true, 2>(a, s) : launch_bf<4, 1, true, true, 2>(a, s);
-        return mw == 2 ? launch_bf<1, 2, true, true, 2>(a, s) : launch_bf<1, 1, true, true, 2>(a, s);
+        if (nt == 4) return mw == 2 ? launch_bf<4, 2, true, true, 2>(a, ws, wsb, s) : launch_bf<4, 1, true, true, 2>(a, ws, wsb, s);
+        return mw == 2 ? launch_bf<1, 2, true, true, 2>(a, ws, wsb, s) : launch_bf<1, 1, true, true, 2>(a, ws, wsb, s);
     } else if constexpr (CI8) {
-        if (nt == 8) return mw == 2 ? launch_bf<8, 2, true, false, 4>(a, s) : launch_bf<8, 1, true, false, 4>(a, s);
-        if (nt == 4) return mw == 2 ? launch_bf<4, 2, true, false, 4>(a, s) : launch_bf<4, 1, true, false, 4>(a, s);
-        return mw == 2 ? launch_bf<1, 2, true, false, 4>(a, s) : launch_bf<1, 1, true, false, 4>(a, s);
+        if (nt == 8) return mw == 2 ? launch_bf<8, 2, true, false, 4>(a, ws, wsb, s) : launch_bf<8, 1, true, false, 4>(a, ws, wsb, s);
+        if (nt == 4) return mw == 2 ? launch_bf<4, 2, true, false, 4>(a, ws, wsb, s) : launch_bf<4, 1, true, false, 4>(a, ws, wsb, s);
+        return mw == 2 ? launch_bf<1, 2, true, false, 4>(a, ws, wsb, s) : launch_bf<1, 1, true, false, 4>(a, ws, wsb, s);
     } else {
-        if (nt == 4) return mw == 2 ? launch_bf<4, 2, false, false, 2>(a, s) : launch_bf<4, 1, false, false, 2>(a, s);
-        return mw == 2 ? launch_bf<1, 2, false, false, 2>(a, s) : launch_bf<1, 1, false, false, 2>(a, s);
+        if (nt == 4) return mw == 2 ? launch_bf<4, 2, false, false, 2>(a, ws, wsb, s) : launch_bf<4, 1, false, false, 2>(a, ws, wsb, s);
+        return mw == 2 ? launch_bf<1, 2, false, false, 2>(a, ws, wsb, s) : launch_bf<1, 1, false, false, 2>(a, ws, wsb, s);
     }
 }
 
 static int run_beamformer(int in_type, int precision, const void* input, const void* phasors,
                           void* output, uint64_t A, uint64_t F, uint64_t T, uint64_t P,
                           uint64_t B, int incoherent, int incoherent_sqrt, uint64_t rate,
-                          int polarize, hipStream_t s) {
+                          int polarize, unsigned char* ws, size_t wsb, hipStream_t s) {
     if (P != 2) return fail("beamformer: number of polarizations must be 2");
     if (in_type != BLADE_CF32 && in_type != BLADE_CI8) return fail("beamformer: input must be CF32 or CI8");
     if (precision != BLADE_PREC_F32 && precision != BLADE_PREC_F16X2) return fail("beamformer: unknown precision");
@@ -590,15 +679,18 @@ static int run_beamformer(int in_type, int precision, const void* input, const v
     a.nsteps = 0;
     a.n_tchunks = 0;
     a.chunks_per_wg = 1;
+    a.pack = nullptr;
+    a.slab_bytes = 0;
+    a.wbytes = 0;
     const int mw = B > 8 ? 2 : 1;
-    if (precision == BLADE_PREC_F16X2) return dispatch<true, true>(a, T % 4 == 0 ? 4 : 1, mw, s);
+    if (precision == BLADE_PREC_F16X2) return dispatch<true, true>(a, T % 4 == 0 ? 4 : 1, mw, ws, wsb, s);
     if (in_type == BLADE_CI8) {
         int nt = T % 8 == 0 ? 8 : (T % 4 == 0 ? 4 : 1);
         const int want = env_int("BLADE_BF_NT", 0);
         if (want > 0 && want < nt && T % want == 0 && (rate == 1 || want % 4 == 0)) nt = want;
-        return dispatch<true, false>(a, nt, mw, s);
+        return dispatch<true, false>(a, nt, mw, ws, wsb, s);
     }
-    return dispatch<false, false>(a, T % 4 == 0 ? 4 : 1, mw, s);
+    return dispatch<false, false>(a, T % 4 == 0 ? 4 : 1, mw, ws, wsb, s);
 }
 
 }  // namespace bl
@@ -610,7 +702,7 @@ extern "C" int blade_beamformer_ata(int in_type, int precision, const void* inpu
                                     uint64_t T, uint64_t P, uint64_t B, int incoherent,
                                     int incoherent_sqrt, blade_stream_t stream) {
     return run_beamformer(in_type, precision, input, phasors, output, A, F, T, P, B, incoherent,
-                          incoherent_sqrt, 1, 0, reinterpret_cast<hipStream_t>(stream));
+                          incoherent_sqrt, 1, 0, nullptr, 0, reinterpret_cast<hipStream_t>(stream));
 }
 
 extern "C" int blade_beamform_chain(int in_type, int precision, const void* input,
@@ -619,5 +711,27 @@ extern "C" int blade_beamform_chain(int in_type, int precision, const void* inpu
                                     int incoherent_sqrt, uint64_t rate, int polarize,
                                     blade_stream_t stream) {
     return run_beamformer(in_type, precision, input, phasors, output, A, F, T, P, B, incoherent,
-                          incoherent_sqrt, rate, polarize, reinterpret_cast<hipStream_t>(stream));
+                          incoherent_sqrt, rate, polarize, nullptr, 0,
+                          reinterpret_cast<hipStream_t>(stream));
+}
+
+extern "C" uint64_t blade_beamformer_workspace_size(int precision, uint64_t A, uint64_t F,
+                                                    uint64_t B) {
+    const bool f16 = precision == BLADE_PREC_F16X2;
+    const int mw = B > 8 ? 2 : 1;
+    size_t wbytes = 0, slab = 0;
+    slab_sizes(f16, mw, nsteps_for(f16, (int)A), &wbytes, &slab);
+    return (uint64_t)slab * F * groups_for((int)B, mw);
+}
+
+extern "C" int blade_beamform_chain_ws(int in_type, int precision, const void* input,
+                                       const void* phasors, void* output, uint64_t A, uint64_t F,
+                                       uint64_t T, uint64_t P, uint64_t B, int incoherent,
+                                       int incoherent_sqrt, uint64_t rate, int polarize,
+                                       void* workspace, uint64_t workspace_bytes,
+                                       blade_stream_t stream) {
+    if (!workspace) return fail("beamformer: null workspace");
+    return run_beamformer(in_type, precision, input, phasors, output, A, F, T, P, B, incoherent,
+                          incoherent_sqrt, rate, polarize, static_cast<unsigned char*>(workspace),
+                          (size_t)workspace_bytes, reinterpret_cast<hipStream_t>(stream));
 }

@@ -284,6 +284,86 @@ __global__ __launch_bounds__(1024) void chan_lds_kernel(const float2* __restrict
     }
 }
 
+// Channelizer with the detector fused on the output: one workgroup per
+// (a, f) walks its S = T/C spectra, transforms both pols in LDS and
+// integrates |X|^2+|Y|^2 (or XX*, YY*, Re XY*, Im XY*) of every fine channel
+// over N consecutive spectra in registers, writing
+//   out[a, f*C + k', s/N, q]   (N <= S, N | S)
+//   out[a, f*C + k', 0, q]    (+=) when N spans several blocks (accumulate).
+// The CF32 spectra never reach HBM.
+template <int NPOL, int KPT>
+__global__ __launch_bounds__(1024) void chan_detect_kernel(const float2* __restrict__ in,
+                                                           float* __restrict__ out, int C,
+                                                           int S, int N, uint64_t n_af,
+                                                           const float2* __restrict__ tw,
+                                                           Radices rad, int accumulate) {
+    extern __shared__ float2 data[];
+    const int PC = 2 * C;
+    const int groups = (N <= S) ? S / N : 1;
+    const int half = C / 2;
+    for (uint64_t af = blockIdx.x; af < n_af; af += gridDim.x) {
+        float acc[KPT][NPOL];
+#pragma unroll
+        for (int j = 0; j < KPT; j++)
+#pragma unroll
+            for (int q = 0; q < NPOL; q++) acc[j][q] = 0.f;
+        for (int s = 0; s < S; s++) {
+            const float4* src = reinterpret_cast<const float4*>(in + ((af * S + s) * (uint64_t)C) * 2);
+            for (int n = threadIdx.x; n < C; n += blockDim.x) {
+                const float4 q = src[n];
+                data[n] = make_float2(q.x, q.y);
+                data[C + n] = make_float2(q.z, q.w);
+            }
+            __syncthreads();
+            int Ns = 1;
+            for (int i = 0; i < rad.n; i++) {
+                const int R = rad.r[i];
+                switch (R) {
+                    case 2: stockham_pass<2>(data, C, PC, Ns, tw); break;
+                    case 3: stockham_pass<3>(data, C, PC, Ns, tw); break;
+                    case 4: stockham_pass<4>(data, C, PC, Ns, tw); break;
+                    case 5: stockham_pass<5>(data, C, PC, Ns, tw); break;
+                    case 7: stockham_pass<7>(data, C, PC, Ns, tw); break;
+                    default: stockham_pass<8>(data, C, PC, Ns, tw); break;
+                }
+                Ns *= R;
+            }
+#pragma unroll
+            for (int j = 0; j < KPT; j++) {
+                const int k = threadIdx.x + j * blockDim.x;
+                if (k < C) {
+                    const float2 X = data[k], Y = data[C + k];
+                    acc[j][0] += X.x * X.x + X.y * X.y + (NPOL == 1 ? Y.x * Y.x + Y.y * Y.y : 0.f);
+                    if constexpr (NPOL == 4) {
+                        acc[j][1] += Y.x * Y.x + Y.y * Y.y;
+                        acc[j][2] += X.x * Y.x + X.y * Y.y;
+                        acc[j][3] += X.y * Y.x - X.x * Y.y;
+                    }
+                }
+            }
+            __syncthreads();
+            const bool flush = (N <= S) ? ((s + 1) % N == 0) : (s == S - 1);
+            if (flush) {
+                const int grp = (N <= S) ? s / N : 0;
+#pragma unroll
+                for (int j = 0; j < KPT; j++) {
+                    const int k = threadIdx.x + j * blockDim.x;
+                    if (k < C) {
+                        int ko = k + half;
+                        if (ko >= C) ko -= C;
+                        float* o = out + ((af * C + ko) * groups + grp) * NPOL;
+#pragma unroll
+                        for (int q = 0; q < NPOL; q++) {
+                            o[q] = accumulate ? o[q] + acc[j][q] : acc[j][q];
+                            acc[j][q] = 0.f;
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
 // Direct O(C^2) DFT for rates with prime factors above 7.
 template <int P>
 __global__ __launch_bounds__(256) void chan_direct_kernel(const float2* __restrict__ in,
@@ -446,4 +526,51 @@ extern "C" int blade_channelizer_plan(uint64_t rate, blade_stream_t stream) {
     if (rate <= 16) return BLADE_SUCCESS;   // register kernels carry their twiddles as arguments
     const float2* tw = nullptr;
     return get_twiddles((int)rate, reinterpret_cast<hipStream_t>(stream), &tw);
+}
+
+extern "C" int blade_channelizer_detector(const void* input, void* output, uint64_t A, uint64_t F,
+                                          uint64_t T, uint64_t rate, uint64_t integration,
+                                          uint64_t out_pols, int accumulate, blade_stream_t stream) {
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (rate == 0 || T % rate) return fail("channelizer-detector: T must be divisible by the rate");
+    if (rate % 2 && rate != 1) return fail("channelizer-detector: the rate should be divisible by 2");
+    if (out_pols != 1 && out_pols != 4) return fail("channelizer-detector: output polarizations must be 1 or 4");
+    const uint64_t S = T / rate;
+    if (integration == 0) return fail("channelizer-detector: integration size should be more than zero");
+    if (integration <= S ? (S % integration != 0) : (integration % S != 0))
+        return fail("channelizer-detector: integration must divide, or be a multiple of, the spectra per block");
+    if (2 * rate > (uint64_t)kMaxLdsComplex) return fail("channelizer-detector: rate above 8192 is not supported");
+    if (A * F * T == 0) return BLADE_SUCCESS;
+    if (!input || !output) return fail("channelizer-detector: null pointer");
+    if ((uintptr_t)input % 16 || (uintptr_t)output % 16) return fail("channelizer-detector: misaligned buffer");
+    const int C = (int)rate;
+    Radices rad;
+    if (!factorize(C, &rad)) return fail("channelizer-detector: rate must factor into 2, 3, 5 and 7");
+    const float2* tw = nullptr;
+    int rc = get_twiddles(C, s, &tw);
+    if (rc) return rc;
+    int threads = 64;
+    while (threads < 1024 && threads * 8 < 2 * C) threads *= 2;
+    const size_t lds = sizeof(float2) * 2 * (size_t)C;
+    const uint64_t n_af = A * F;
+    uint64_t blocks = n_af;
+    if (blocks > 256 * 64) blocks = 256 * 64;
+    const void* fn = out_pols == 1 ? reinterpret_cast<const void*>(&chan_detect_kernel<1, 8>)
+                                   : reinterpret_cast<const void*>(&chan_detect_kernel<4, 8>);
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) {
+            set_error(std::string("channelizer-detector: LDS attribute: ") + hipGetErrorString(e));
+            return BLADE_HIP_ERROR;
+        }
+    }
+    const float2* in = static_cast<const float2*>(input);
+    float* out = static_cast<float*>(output);
+    if (out_pols == 1)
+        hipLaunchKernelGGL((chan_detect_kernel<1, 8>), dim3((unsigned)blocks), dim3(threads), lds, s, in, out, C,
+                           (int)S, (int)integration, n_af, tw, rad, accumulate);
+    else
+        hipLaunchKernelGGL((chan_detect_kernel<4, 8>), dim3((unsigned)blocks), dim3(threads), lds, s, in, out, C,
+                           (int)S, (int)integration, n_af, tw, rad, accumulate);
+    return check_launch("channelizer-detector");
 }

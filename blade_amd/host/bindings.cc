@@ -46,6 +46,14 @@ py::dtype numpy_dtype() {
     else return py::dtype::of<T>();
 }
 
+// Bounds-checked indexing: Python's sequence protocol (list(shape),
+// iteration) stops at IndexError.
+template <typename Shape>
+U64 shape_at(const Shape& s, U64 i) {
+    if (i >= Shape::dimensions()) throw py::index_error("shape index out of range");
+    return s[i];
+}
+
 template <typename Shape>
 std::string shape_str(const Shape& s) {
     std::ostringstream os;
@@ -451,7 +459,7 @@ PYBIND11_MODULE(_blade_ext, m) {
         .def_property_readonly("number_of_frequency_channels", &ArrayShape::numberOfFrequencyChannels)
         .def_property_readonly("number_of_time_samples", &ArrayShape::numberOfTimeSamples)
         .def_property_readonly("number_of_polarizations", &ArrayShape::numberOfPolarizations)
-        .def("__getitem__", [](ArrayShape& s, U64 i) { return s[i]; })
+        .def("__getitem__", [](ArrayShape& s, U64 i) { return shape_at(s, i); })
         .def("__len__", [](ArrayShape&) { return 4; })
         .def("__repr__", [](ArrayShape& s) { return "ArrayShape(shape=" + shape_str(s) + ")"; });
     py::class_<PhasorShape>(mem, "phasor_shape")
@@ -461,19 +469,19 @@ PYBIND11_MODULE(_blade_ext, m) {
         .def_property_readonly("number_of_frequency_channels", &PhasorShape::numberOfFrequencyChannels)
         .def_property_readonly("number_of_time_samples", &PhasorShape::numberOfTimeSamples)
         .def_property_readonly("number_of_polarizations", &PhasorShape::numberOfPolarizations)
-        .def("__getitem__", [](PhasorShape& s, U64 i) { return s[i]; })
+        .def("__getitem__", [](PhasorShape& s, U64 i) { return shape_at(s, i); })
         .def("__len__", [](PhasorShape&) { return 5; })
         .def("__repr__", [](PhasorShape& s) { return "PhasorShape(shape=" + shape_str(s) + ")"; });
     py::class_<DelayShape>(mem, "delay_shape")
         .def(py::init<const DelayShape::Type&>(), "shape"_a)
         .def_property_readonly("number_of_beams", &DelayShape::numberOfBeams)
         .def_property_readonly("number_of_antennas", &DelayShape::numberOfAntennas)
-        .def("__getitem__", [](DelayShape& s, U64 i) { return s[i]; })
+        .def("__getitem__", [](DelayShape& s, U64 i) { return shape_at(s, i); })
         .def("__len__", [](DelayShape&) { return 2; })
         .def("__repr__", [](DelayShape& s) { return "DelayShape(shape=" + shape_str(s) + ")"; });
     py::class_<VectorShape>(mem, "vector_shape")
         .def(py::init<const VectorShape::Type&>(), "shape"_a)
-        .def("__getitem__", [](VectorShape& s, U64 i) { return s[i]; })
+        .def("__getitem__", [](VectorShape& s, U64 i) { return shape_at(s, i); })
         .def("__len__", [](VectorShape&) { return 1; })
         .def("__repr__", [](VectorShape& s) { return "VectorShape(shape=" + shape_str(s) + ")"; });
     bind_device<Device::CPU>(mem, "cpu");

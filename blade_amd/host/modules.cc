@@ -44,12 +44,18 @@ Generic<IT, OT>::Generic(const Config& config, const Input& input, const Stream&
 template <typename IT, typename OT>
 Result Generic<IT, OT>::process(const U64&, const Stream& stream) {
     const auto& s = getInputBuffer().shape();
-    BL_ABI_CHECK(blade_beamformer_ata(
-        TypeInfo<IT>::abi, config.enableSplitFp16 ? BLADE_PREC_F16X2 : BLADE_PREC_F32,
-        input.buf.data(), input.phasors.data(), output.buf.data(), s.numberOfAspects(),
-        s.numberOfFrequencyChannels(), s.numberOfTimeSamples(), s.numberOfPolarizations(),
-        getInputPhasors().shape().numberOfBeams(), config.enableIncoherentBeam,
-        config.enableIncoherentBeamSqrt, abi(stream)));
+    const int prec = config.enableSplitFp16 ? BLADE_PREC_F16X2 : BLADE_PREC_F32;
+    const U64 B = getInputPhasors().shape().numberOfBeams();
+    if (workspace.empty()) {
+        const U64 bytes = blade_beamformer_workspace_size(prec, s.numberOfAspects(),
+                                                          s.numberOfFrequencyChannels(), B);
+        workspace = Vector<Device::CUDA, U8, VectorShape>({bytes});
+    }
+    BL_ABI_CHECK(blade_beamform_chain_ws(
+        TypeInfo<IT>::abi, prec, input.buf.data(), input.phasors.data(), output.buf.data(),
+        s.numberOfAspects(), s.numberOfFrequencyChannels(), s.numberOfTimeSamples(),
+        s.numberOfPolarizations(), B, config.enableIncoherentBeam, config.enableIncoherentBeamSqrt,
+        1, 0, workspace.data(), workspace.size(), abi(stream)));
     return Result::SUCCESS;
 }
 
@@ -84,6 +90,11 @@ ATA<IT, OT>::ATA(const typename Generic<IT, OT>::Config& config,
         throw Result::ERROR;
     }
     this->output.buf = ArrayTensor<Device::CUDA, OT>(this->getOutputBufferShape());
+    // Allocated here so that process() allocates nothing (hipGraph capture).
+    const U64 wsBytes = blade_beamformer_workspace_size(
+        config.enableSplitFp16 ? BLADE_PREC_F16X2 : BLADE_PREC_F32, in.numberOfAspects(),
+        in.numberOfFrequencyChannels(), ph.numberOfBeams());
+    this->workspace = Vector<Device::CUDA, U8, VectorShape>({wsBytes});
     BL_INFO("Shape: {} -> {}", in, this->getOutputBuffer().shape());
 }
 

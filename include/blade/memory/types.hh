@@ -83,6 +83,7 @@ struct TypeInfo;
         inline static const U64 components = CPLX ? 2 : 1;             \
     };
 
+BLADE_TYPEINFO(U8, U8, false, "U8", BLADE_I8)
 BLADE_TYPEINFO(I8, I8, false, "I8", BLADE_I8)
 BLADE_TYPEINFO(F16, F16, false, "F16", BLADE_F16)
 BLADE_TYPEINFO(F32, F32, false, "F32", BLADE_F32)

@@ -47,6 +47,8 @@ class BLADE_API Generic : public Module {
     const Config config;
     const Input input;
     Output output;
+    // Packed matrix-core operands of the phasors (rebuilt every process()).
+    Vector<Device::CUDA, U8, VectorShape> workspace;
 
     virtual const ArrayShape getOutputBufferShape() const = 0;
 };

@@ -87,6 +87,23 @@ int blade_beamform_chain(int in_type, int precision, const void* input, const vo
                          int polarize, blade_stream_t stream);
 
 /*
+ * The two calls above re-pack the phasors into the matrix-core operand
+ * layout on every call, into a library-owned buffer per (device, stream).
+ * Callers that own their memory (modules, hipGraph capture) size a device
+ * workspace with blade_beamformer_workspace_size and pass it to
+ * blade_beamform_chain_ws (same semantics as blade_beamform_chain).
+ * The buffer is what the reference keeps in managed phasor memory per
+ * Phasor module (src/modules/phasor/ata.cc:96); no reference counterpart
+ * exists for the packed layout itself.
+ */
+uint64_t blade_beamformer_workspace_size(int precision, uint64_t A, uint64_t F, uint64_t B);
+int blade_beamform_chain_ws(int in_type, int precision, const void* input, const void* phasors,
+                            void* output, uint64_t A, uint64_t F, uint64_t T, uint64_t P,
+                            uint64_t B, int incoherent, int incoherent_sqrt, uint64_t rate,
+                            int polarize, void* workspace, uint64_t workspace_bytes,
+                            blade_stream_t stream);
+
+/*
  * Channelizer — fftshift(FFT_C) of every C-sample chunk along T.
  * Replaces Modules::Channelizer<CF32,CF32>::process (src/modules/channelizer/base.cc:122-146,
  * pre-kernel channelizer.cu:5-19 and cuFFT C2C forward) with the general-rate
@@ -105,6 +122,21 @@ int blade_channelizer(const void* input, void* output, uint64_t A, uint64_t F, u
  * (src/modules/channelizer/base.cc:58-93).
  */
 int blade_channelizer_plan(uint64_t rate, blade_stream_t stream);
+
+/*
+ * Channelizer followed by Detector in one pass (P = 2): the ModeH chain
+ * (include/blade/bundles/generic/mode_h.hh:55-106: Channelizer -> Detector)
+ * without the CF32 spectra ever reaching memory.  Per fine channel
+ *   out[a, f*C+k', g, q] = sum over N consecutive spectra of |X|^2+|Y|^2
+ *                          (out_pols 1) or [XX*, YY*, Re XY*, Im XY*] (4),
+ * as Detector (src/modules/detector/detector.cu:7-51) applied to the
+ * Channelizer output.  N divides S = T/rate (output [A, F*C, S/N, out_pols]),
+ * or is a multiple of S (output [A, F*C, 1, out_pols], accumulate != 0 adds
+ * this block to it: stepped integration across blocks).
+ */
+int blade_channelizer_detector(const void* input, void* output, uint64_t A, uint64_t F,
+                               uint64_t T, uint64_t rate, uint64_t integration,
+                               uint64_t out_pols, int accumulate, blade_stream_t stream);
 
 /*
  * Polarizer XY -> LR: out[...,0] = X + jY, out[...,1] = X - jY.

@@ -342,7 +342,9 @@ def modeb_config(in_shape, out_shape, cal, rate, circ, det):
 def test_mode_b_bundle(it, rate, circ):
     # tests/bundles/ata/mode_b.py, with the module's own delays feeding the
     # oracle phasors (the delay engine itself is checked separately).
-    A, F, T = 6, 16, 512
+    # T / rate must stay a multiple of the beamformer block size (512), as in
+    # the reference (src/modules/beamformer/generic.cc:17-21).
+    A, F, T = 6, 16, 512 * rate
     in_shape = (A, F, T, 2)
     out_shape = (3, F * rate, T // rate, 1)
     rng = np.random.default_rng(5)

@@ -109,6 +109,32 @@ def test_beamform_chain(rate, polarize, precision):
     assert O.norm_rel_err(y, ref) <= TOL
 
 
+@pytest.mark.parametrize("precision", ["f32", "f16x2"])
+def test_beamform_chain_caller_workspace(precision):
+    # blade_beamform_chain_ws with a caller-owned workspace gives the same bits
+    # as the library-owned one, on two calls in a row (the second smaller).
+    import ctypes as ct
+    from blade_amd import _lib
+    rng = np.random.default_rng(31)
+    for (A, F, T, B) in ((28, 6, 2048, 16), (20, 3, 1024, 8)):
+        x8 = dev(O.ci8_gaussian((A, F, T, 2), rng=rng))
+        w = dev(phasors_for(B, A, F, rng).astype(np.complex64))
+        ref = host(ops.beamform_chain(x8, w, True, False, 4, True, precision))
+        prec = {"f32": 0, "f16x2": 1}[precision]
+        lib = _lib.load()
+        nbytes = lib.blade_beamformer_workspace_size(prec, A, F, B)
+        ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        out = torch.empty_like(torch.from_numpy(ref)).to("cuda")
+        _lib.call("blade_beamform_chain_ws", _lib.CI8, prec, ct.c_void_p(x8.data_ptr()),
+                  ct.c_void_p(w.data_ptr()), ct.c_void_p(out.data_ptr()), A, F, T, 2, B, 1, 0, 4, 1,
+                  ct.c_void_p(ws.data_ptr()), nbytes, ops.stream_handle())
+        assert np.array_equal(host(out), ref)
+        with pytest.raises(BladeError, match="workspace"):
+            _lib.call("blade_beamform_chain_ws", _lib.CI8, prec, ct.c_void_p(x8.data_ptr()),
+                      ct.c_void_p(w.data_ptr()), ct.c_void_p(out.data_ptr()), A, F, T, 2, B, 1, 0, 4, 1,
+                      ct.c_void_p(ws.data_ptr()), nbytes - 1, ops.stream_handle())
+
+
 def test_beamformer_rejects_bad_shapes():
     x = torch.zeros((2, 2, 8, 2), dtype=torch.complex64, device="cuda")
     w = torch.zeros((1, 2, 2, 1, 2), dtype=torch.complex64, device="cuda")
@@ -154,6 +180,30 @@ def test_channelizer_in_place_rate_equals_t():
     t = dev(x)
     ops.channelizer(t, 1024, out=t)
     assert O.norm_rel_err(host(t).reshape(2, 3 * 1024, 1, 2), O.channelizer(x, 1024)) <= TOL
+
+
+@pytest.mark.parametrize("A,F,T,C,N,npol", [(2, 3, 16384, 8192, 2, 1),     # config 2 shape per block
+                                            (2, 3, 16384, 8192, 1, 4),
+                                            (1, 4, 4096, 1024, 2, 1),
+                                            (2, 2, 2400, 200, 3, 4)])
+def test_channelizer_detector(A, F, T, C, N, npol):
+    rng = np.random.default_rng(C + N)
+    x = crand(rng, A, F, T, 2)
+    y = host(ops.channelizer_detector(dev(x), C, N, npol))
+    ref = O.detector(O.channelizer(x, C), N, npol)
+    assert y.shape == ref.shape
+    assert O.norm_rel_err(y, ref) <= TOL
+
+
+def test_channelizer_detector_stepped():
+    # 4 spectra integrated over two blocks of 2 (config 2's Stokes-I).
+    rng = np.random.default_rng(77)
+    blocks = [crand(rng, 2, 3, 4096, 2) for _ in range(2)]
+    out = torch.empty((2, 3 * 2048, 1, 1), dtype=torch.float32, device="cuda")
+    ops.channelizer_detector(dev(blocks[0]), 2048, 4, 1, out=out, accumulate=False)
+    ops.channelizer_detector(dev(blocks[1]), 2048, 4, 1, out=out, accumulate=True)
+    ref = O.detector_stepped([O.channelizer(b, 2048) for b in blocks], 4, 1)
+    assert O.norm_rel_err(host(out), ref) <= TOL
 
 
 def test_channelizer_rate_one_is_copy():
