@@ -10,8 +10,9 @@
 // with M = 2B (rows: beam, re/im), K = 2A (antenna, re/im), N = T.  Phasors are
 // the A operand (staged once per workgroup in LDS), voltages the B operand,
 // read straight from HBM in their native [A,F,T,P] layout: each lane owns NT
-// consecutive time samples of one antenna, so one 16-byte load (CI8) feeds
-// 2 components x 2 pols x NT x MW MFMAs.  The K index is ordered so that the
+// time samples of one antenna in runs of 4 consecutive samples (one 16-byte
+// CI8 load feeds 2 components x 2 pols x 4 x MW MFMAs), and the 16 lanes of a
+// group read 256 contiguous bytes per load.  The K index is ordered so that the
 // four 16-lane groups of the wavefront read DIFFERENT antennas: no voltage is
 // fetched twice.
 //
@@ -93,59 +94,59 @@ struct BfArgs {
 //   split FP16 [m][p][step][hl][lane] halfx8 (8 k values = 4 antennas x re/im),
 //              components scaled by a per-pol power of two so that all are < 1,
 //              hi = fp16(w), lo = fp16(w - hi).
+// One workgroup per (pol p, beam tile m) of one (f, beam group): grid
+// (MW * 2, F, groups).  The split-FP16 scale is shared by the whole beam group
+// of a pol, so that path stages all MW * 8 beams of the pol for the maximum.
 template <int MW, bool F16X2>
 __global__ __launch_bounds__(256) void bf_pack_kernel(BfArgs a) {
     constexpr int AK = F16X2 ? 4 * KG : KG;
-    __shared__ unsigned int s_max[2];
+    constexpr int NB = MW * (TILE / 2);              // beams of one group
+    constexpr int SB = F16X2 ? NB : TILE / 2;        // beams staged in LDS
+    extern __shared__ float2 s_w[];                  // [SB][A] phasors of pol p
+    __shared__ unsigned int s_max;
+    const int p = blockIdx.x & 1;
+    const int m = blockIdx.x >> 1;
     const int f = blockIdx.y;
-    const int b_wg = blockIdx.z * MW * (TILE / 2);
+    const int A = a.A;
     const int nsteps = a.nsteps;
+    const int b0 = blockIdx.z * NB + (F16X2 ? 0 : m * (TILE / 2));   // first staged beam
     unsigned char* slab = a.pack + ((size_t)f * gridDim.z + blockIdx.z) * a.slab_bytes;
     float* tail = reinterpret_cast<float*>(slab + a.wbytes);
-    if (threadIdx.x < 2) s_max[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) s_max = 0u;
+    float mx = 0.f;
+    for (int e = threadIdx.x; e < SB * A; e += blockDim.x) {
+        const int bl_ = e / A, an = e - bl_ * A, b = b0 + bl_;
+        float2 w = make_float2(0.f, 0.f);
+        if (b < a.B) w = a.w[(((size_t)b * A + an) * a.F + f) * 2 + p];
+        s_w[e] = w;
+        mx = fmaxf(mx, fmaxf(fabsf(w.x), fabsf(w.y)));
+    }
     __syncthreads();
-    float unscale[2] = {1.f, 1.f};
+    // beam (within the staged rows) of A-operand row `row` of tile m
+    const int row_beam0 = F16X2 ? m * (TILE / 2) : 0;
+    float unscale = 1.f;
     if constexpr (F16X2) {
-        float mx[2] = {0.f, 0.f};
-        for (int e = threadIdx.x; e < MW * (TILE / 2) * a.A; e += blockDim.x) {
-            const int bl_ = e / a.A, an = e - bl_ * a.A, b = b_wg + bl_;
-            if (b < a.B) {
-                const float4 w = reinterpret_cast<const float4*>(a.w)[((size_t)b * a.A + an) * a.F + f];
-                mx[0] = fmaxf(mx[0], fmaxf(fabsf(w.x), fabsf(w.y)));
-                mx[1] = fmaxf(mx[1], fmaxf(fabsf(w.z), fabsf(w.w)));
-            }
-        }
-        atomicMax(&s_max[0], __float_as_uint(mx[0]));
-        atomicMax(&s_max[1], __float_as_uint(mx[1]));
+        atomicMax(&s_max, __float_as_uint(mx));
         __syncthreads();
-        float scale[2];
-#pragma unroll
-        for (int p = 0; p < 2; p++) {
-            int ex = 0;
-            frexpf(__uint_as_float(s_max[p]), &ex);
-            scale[p] = ldexpf(1.0f, -ex);
-            unscale[p] = ldexpf(1.0f, ex);
-        }
-        halfx8* wh = reinterpret_cast<halfx8*>(slab);
-        const int total = MW * 2 * nsteps * 2 * 64;
-        for (int e = threadIdx.x; e < total; e += blockDim.x) {
-            const int ln = e & 63;
-            const int hl = (e >> 6) & 1;
-            const int r = e >> 7;                 // (m * 2 + p) * nsteps + st
-            const int st = r % nsteps;
-            const int p = (r / nsteps) & 1;
-            const int m = (r / nsteps) >> 1;
-            const int row = ln % TILE, kg = ln / TILE;
-            const int b = b_wg + m * (TILE / 2) + row / 2, co = row & 1;
+        int ex = 0;
+        frexpf(__uint_as_float(s_max), &ex);
+        const float scale = ldexpf(1.0f, -ex);
+        unscale = ldexpf(1.0f, ex);
+        // [m][p][step][hl][lane] halfx8, this workgroup's (m, p) block
+        halfx8* wh = reinterpret_cast<halfx8*>(slab) + (size_t)((m * 2 + p) * nsteps) * 128;
+        for (int e = threadIdx.x; e < nsteps * 128; e += blockDim.x) {
+            const int st = e >> 7, hl = (e >> 6) & 1, ln = e & 63;
+            const int row = ln & (TILE - 1), kg = ln >> 4;
+            const int bl_ = row_beam0 + (row >> 1), co = row & 1;
             halfx8 h;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int an = st * AK + 4 * kg + q;
                 float wr = 0.f, wi = 0.f;
-                if (b < a.B && an < a.A) {
-                    const float2 w = a.w[(((size_t)b * a.A + an) * a.F + f) * 2 + p];
-                    wr = w.x * scale[p];
-                    wi = w.y * scale[p];
+                if (an < A) {
+                    const float2 w = s_w[bl_ * A + an];
+                    wr = w.x * scale;
+                    wi = w.y * scale;
                 }
                 // k = (antenna, c): c = 0 multiplies Re x, c = 1 multiplies Im x.
                 const float v0 = co == 0 ? wr : wi;
@@ -162,35 +163,35 @@ __global__ __launch_bounds__(256) void bf_pack_kernel(BfArgs a) {
             wh[e] = h;
         }
     } else {
-        float* wl = reinterpret_cast<float*>(slab);
-        const int total = MW * 2 * nsteps * 64;
-        for (int e = threadIdx.x; e < total; e += blockDim.x) {
-            const int row = e % TILE;
-            const int kg = (e / TILE) % KG;
-            const int r = e / 64;                 // (m * 2 + p) * nsteps + st
-            const int st = r % nsteps;
-            const int p = (r / nsteps) & 1;
-            const int m = (r / nsteps) >> 1;
-            const int an = (st / 2) * KG + kg, c = st & 1;
-            const int b = b_wg + m * (TILE / 2) + row / 2, co = row & 1;
+        // [m][p][step][g][row] float, step = 2 * antenna group + component
+        float* wl = reinterpret_cast<float*>(slab) + (size_t)((m * 2 + p) * nsteps) * 64;
+        for (int e = threadIdx.x; e < nsteps * 64; e += blockDim.x) {
+            const int st = e >> 6, kg = (e >> 4) & 3, row = e & (TILE - 1);
+            const int an = (st >> 1) * KG + kg, c = st & 1;
+            const int bl_ = row >> 1, co = row & 1;
             float v = 0.f;
-            if (b < a.B && an < a.A) {
-                const float2 w = a.w[(((size_t)b * a.A + an) * a.F + f) * 2 + p];
+            if (an < A) {
+                const float2 w = s_w[bl_ * A + an];
                 v = (co == 0) ? (c == 0 ? w.x : -w.y) : (c == 0 ? w.y : w.x);
             }
             wl[e] = v;
         }
     }
-    for (int e = threadIdx.x; e < 2 * 128; e += blockDim.x) {
-        const int p = e / 128, an = e % 128;
-        float v = 0.f;
-        if (an < a.A) {
-            const float2 w = a.w[((size_t)an * a.F + f) * 2 + p];   // beam 0
-            v = w.x * w.x + w.y * w.y;
+    if (m == 0) {
+        // |w0|^2 of pol p (read only by the beam-group-0 workgroups, whose
+        // staged row 0 is beam 0) and the unscale of pol p.
+        if (blockIdx.z == 0) {
+            for (int an = threadIdx.x; an < 128; an += blockDim.x) {
+                float v = 0.f;
+                if (an < A) {
+                    const float2 w = s_w[an];
+                    v = w.x * w.x + w.y * w.y;
+                }
+                tail[p * 128 + an] = v;
+            }
         }
-        tail[e] = v;
+        if (threadIdx.x == 0) tail[256 + p] = unscale;
     }
-    if (threadIdx.x < 2) tail[256 + threadIdx.x] = unscale[threadIdx.x];
 }
 
 __device__ __forceinline__ void polarize2(float2& X, float2& Y) {
@@ -200,12 +201,41 @@ __device__ __forceinline__ void polarize2(float2& X, float2& Y) {
     Y = R;
 }
 
-// Epilogue: beam b, both pols, NT consecutive time samples from t0; writes
-// the plain beam, or spectra of `rate` samples (fftshifted) into
-// [B', F*C, T/C, P], optionally polarized.
+// Time sample of a lane's n-th column: lanes own runs of RUN = min(NT, 4)
+// consecutive samples, the runs of one lane TILE * RUN samples apart, so a
+// 16-lane group reads and writes TILE * RUN contiguous samples per access.
 template <int NT>
+struct TimeMap {
+    static constexpr int RUN = NT < 4 ? NT : 4;
+    // offset of column n from the lane's first sample (chunk base + RUN * i)
+    static __device__ __forceinline__ constexpr int off(int n) {
+        return (n / RUN) * (TILE * RUN) + n % RUN;
+    }
+};
+
+// One 16-byte output store (both pols of one sample); nontemporal on request.
+template <bool NTS>
+__device__ __forceinline__ void store4(float4* p, float2 X, float2 Y) {
+    if constexpr (NTS) {
+        // Component-wise: merged into one global_store_dwordx4 ... nt (a
+        // vector operand here would be lowered through private memory).
+        float* q = reinterpret_cast<float*>(p);
+        __builtin_nontemporal_store(X.x, q);
+        __builtin_nontemporal_store(X.y, q + 1);
+        __builtin_nontemporal_store(Y.x, q + 2);
+        __builtin_nontemporal_store(Y.y, q + 3);
+    } else {
+        *p = make_float4(X.x, X.y, Y.x, Y.y);
+    }
+}
+
+// Epilogue: beam b, both pols, the NT samples of a lane whose first sample is
+// t0 (TimeMap); writes the plain beam, or spectra of `rate` samples
+// (fftshifted) into [B', F*C, T/C, P], optionally polarized.
+template <int NT, bool NTS>
 __device__ __forceinline__ void store_beam(const BfArgs& a, int b, int f, int t0,
                                            float2 (&v)[2][NT]) {
+    typedef TimeMap<NT> M;
     if (a.rate == 1) {
         if (a.polarize) {
 #pragma unroll
@@ -213,7 +243,7 @@ __device__ __forceinline__ void store_beam(const BfArgs& a, int b, int f, int t0
         }
         float4* dst = reinterpret_cast<float4*>(a.y) + ((size_t)b * a.F + f) * a.T + t0;
 #pragma unroll
-        for (int n = 0; n < NT; n++) dst[n] = make_float4(v[0][n].x, v[0][n].y, v[1][n].x, v[1][n].y);
+        for (int n = 0; n < NT; n++) store4<NTS>(dst + M::off(n), v[0][n], v[1][n]);
         return;
     }
     if constexpr (NT % 4 == 0) {
@@ -234,26 +264,26 @@ __device__ __forceinline__ void store_beam(const BfArgs& a, int b, int f, int t0
                     o[p][2] = csub(s02, s13);
                     o[p][3] = csub(d02, d13);
                 }
-                const int s = t0 / 4 + h;
+                const int s = (t0 + M::off(4 * h)) / 4;
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     float2 X = o[0][k], Y = o[1][k];
                     if (a.polarize) polarize2(X, Y);
                     const int kp = (k + 2) & 3;   // fftshift
-                    base[(((size_t)b * a.F + f) * 4 + kp) * S + s] = make_float4(X.x, X.y, Y.x, Y.y);
+                    store4<NTS>(base + (((size_t)b * a.F + f) * 4 + kp) * S + s, X, Y);
                 }
             }
         } else {   // rate 2
             const int S = a.T / 2;
 #pragma unroll
             for (int h = 0; h < NT / 2; h++) {
-                const int s = t0 / 2 + h;
+                const int s = (t0 + M::off(2 * h)) / 2;
 #pragma unroll
                 for (int k = 0; k < 2; k++) {
                     float2 X = k == 0 ? cadd(v[0][2 * h], v[0][2 * h + 1]) : csub(v[0][2 * h], v[0][2 * h + 1]);
                     float2 Y = k == 0 ? cadd(v[1][2 * h], v[1][2 * h + 1]) : csub(v[1][2 * h], v[1][2 * h + 1]);
                     if (a.polarize) polarize2(X, Y);
-                    base[(((size_t)b * a.F + f) * 2 + (k ^ 1)) * S + s] = make_float4(X.x, X.y, Y.x, Y.y);
+                    store4<NTS>(base + (((size_t)b * a.F + f) * 2 + (k ^ 1)) * S + s, X, Y);
                 }
             }
         }
@@ -292,7 +322,7 @@ __device__ __forceinline__ void load_words(uint32_t* dst, const uint32_t* src) {
 // walks `chunks_per_wg` consecutive chunks of WAVES*TILE*NT samples with the
 // phasors staged once; voltage loads run D units ahead of the MFMAs through
 // a register ring that crosses chunk boundaries.
-template <int NT, int MW, bool CI8, bool F16X2, int D>
+template <int NT, int MW, bool CI8, bool F16X2, int D, bool NTS>
 __global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
     constexpr int AK = F16X2 ? 4 * KG : KG;       // antennas per unit
     typedef Unit<NT, CI8, F16X2> U_t;
@@ -317,7 +347,9 @@ __global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
     const int J = nch * upad;
     const size_t ant_stride = (size_t)a.F * a.T;   // complex samples between antennas
 
-    auto chunk_t0 = [&](int c) { return ((cw0 + c) * WAVES + wave) * (TILE * NT) + NT * i; };
+    typedef TimeMap<NT> M;
+    // First sample of this lane in chunk c (runs of M::RUN samples, TimeMap).
+    auto chunk_t0 = [&](int c) { return ((cw0 + c) * WAVES + wave) * (TILE * NT) + M::RUN * i; };
 
     // Loads are unconditional so that the wait-count analysis keeps them in
     // flight across the loop: indices past the end re-load the last unit,
@@ -330,19 +362,26 @@ __global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
         int t0 = chunk_t0(c);
         t0 = (t0 < a.T) ? t0 : 0;
         const size_t off = (size_t)f * a.T + t0;
+        constexpr int R = M::RUN;
         if constexpr (F16X2) {
             const uint32_t* x = static_cast<const uint32_t*>(a.x);
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int an = min(un * AK + 4 * g + q, a.A - 1);
-                load_words<NT>(u.w + q * NT, x + an * ant_stride + off);
+#pragma unroll
+                for (int h = 0; h < NT / R; h++)
+                    load_words<R>(u.w + q * NT + h * R, x + an * ant_stride + off + M::off(h * R));
             }
         } else {
             const int an = min(un * KG + g, a.A - 1);
-            if constexpr (CI8) {
-                load_words<NT>(u.w, static_cast<const uint32_t*>(a.x) + an * ant_stride + off);
-            } else {
-                load_words<4 * NT>(u.w, static_cast<const uint32_t*>(a.x) + 4 * (an * ant_stride + off));
+#pragma unroll
+            for (int h = 0; h < NT / R; h++) {
+                if constexpr (CI8) {
+                    load_words<R>(u.w + h * R, static_cast<const uint32_t*>(a.x) + an * ant_stride + off + M::off(h * R));
+                } else {
+                    load_words<4 * R>(u.w + 4 * h * R,
+                                      static_cast<const uint32_t*>(a.x) + 4 * (an * ant_stride + off + M::off(h * R)));
+                }
             }
         }
     };
@@ -501,7 +540,7 @@ __global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
 #pragma unroll
                             for (int n = 0; n < NT; n++)
                                 v[p][n] = make_float2(acc[m][p][n][r] * unscale[p], acc[m][p][n][r + 1] * unscale[p]);
-                        store_beam<NT>(a, b, fo, t0, v);
+                        store_beam<NT, NTS>(a, b, fo, t0, v);
                     }
                 }
             }
@@ -521,20 +560,22 @@ __global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
 #pragma unroll
                     for (int n = 0; n < NT; n++)
                         v[p][n] = make_float2(a.inc_sqrt ? sqrtf(inc[p][n]) : inc[p][n], 0.f);
-                store_beam<NT>(a, a.B, fo, t0, v);
+                store_beam<NT, NTS>(a, a.B, fo, t0, v);
             }
         }
     };
 
     // ---- Main loop: D units per step, loads run D units ahead. -------------
+    // A ring slot is consumed before it is reloaded, so every slot keeps its
+    // registers across iterations (no copies, no drain of the loads in
+    // flight at the loop latch).
     int j = 0;   // global unit index (chunk-major)
     for (int c = 0; c < nch; c++) {
         for (int ub = 0; ub < upad; ub += D) {
 #pragma unroll
             for (int d = 0; d < D; d++) {
-                const U_t cur = ring[d];
+                if (ub + d < n_units) compute(ring[d], ub + d);
                 load_unit(j + d + D, ring[d]);
-                if (ub + d < n_units) compute(cur, ub + d);
             }
             j += D;
         }
@@ -591,7 +632,7 @@ static int library_workspace(size_t bytes, hipStream_t s, unsigned char** out) {
     return BLADE_SUCCESS;
 }
 
-template <int NT, int MW, bool CI8, bool F16X2, int D>
+template <int NT, int MW, bool CI8, bool F16X2, int D, bool NTS>
 static int launch_bf(const BfArgs& args, unsigned char* ws, size_t ws_bytes, hipStream_t s) {
     BfArgs a = args;
     a.nsteps = nsteps_for(F16X2, a.A);
@@ -618,7 +659,7 @@ static int launch_bf(const BfArgs& args, unsigned char* ws, size_t ws_bytes, hip
     }
     a.chunks_per_wg = cpw;
     dim3 grid((unsigned)((a.n_tchunks + cpw - 1) / cpw), (unsigned)a.F, (unsigned)groups);
-    const void* fn = reinterpret_cast<const void*>(&beamform_kernel<NT, MW, CI8, F16X2, D>);
+    const void* fn = reinterpret_cast<const void*>(&beamform_kernel<NT, MW, CI8, F16X2, D, NTS>);
     if (lds > 150 * 1024) return fail("beamformer: too many antennas for the LDS phasor stage");
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -627,24 +668,40 @@ static int launch_bf(const BfArgs& args, unsigned char* ws, size_t ws_bytes, hip
             return BLADE_HIP_ERROR;
         }
     }
-    hipLaunchKernelGGL((bf_pack_kernel<MW, F16X2>), dim3(1, (unsigned)a.F, (unsigned)groups), dim3(256), 0, s, a);
-    hipLaunchKernelGGL((beamform_kernel<NT, MW, CI8, F16X2, D>), grid, dim3(64 * WAVES), lds, s, a);
+    const size_t pack_lds = (size_t)(F16X2 ? MW : 1) * (TILE / 2) * a.A * sizeof(float2);
+    if (pack_lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&bf_pack_kernel<MW, F16X2>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)pack_lds);
+        if (e != hipSuccess) {
+            set_error(std::string("beamformer: pack LDS attribute: ") + hipGetErrorString(e));
+            return BLADE_HIP_ERROR;
+        }
+    }
+    hipLaunchKernelGGL((bf_pack_kernel<MW, F16X2>), dim3(2 * MW, (unsigned)a.F, (unsigned)groups), dim3(256),
+                       pack_lds, s, a);
+    hipLaunchKernelGGL((beamform_kernel<NT, MW, CI8, F16X2, D, NTS>), grid, dim3(64 * WAVES), lds, s, a);
     return check_launch("beamformer");
 }
 
 // Variant table: (time samples per lane, beam tiles per wave) by precision.
+// Beams go out with nontemporal stores (they are not re-read by this kernel);
+// BLADE_BF_NTSTORE=0 selects plain stores for the FP32 CI8 variant.
 template <bool CI8, bool F16X2>
 static int dispatch(const BfArgs& a, int nt, int mw, unsigned char* ws, size_t wsb, hipStream_t s) {
     if constexpr (F16X2) {
-        if (nt == 4) return mw == 2 ? launch_bf<4, 2, true, true, 2>(a, ws, wsb, s) : launch_bf<4, 1, true, true, 2>(a, ws, wsb, s);
-        return mw == 2 ? launch_bf<1, 2, true, true, 2>(a, ws, wsb, s) : launch_bf<1, 1, true, true, 2>(a, ws, wsb, s);
+        if (nt == 4) return mw == 2 ? launch_bf<4, 2, true, true, 2, true>(a, ws, wsb, s) : launch_bf<4, 1, true, true, 2, true>(a, ws, wsb, s);
+        return mw == 2 ? launch_bf<1, 2, true, true, 2, true>(a, ws, wsb, s) : launch_bf<1, 1, true, true, 2, true>(a, ws, wsb, s);
     } else if constexpr (CI8) {
-        if (nt == 8) return mw == 2 ? launch_bf<8, 2, true, false, 4>(a, ws, wsb, s) : launch_bf<8, 1, true, false, 4>(a, ws, wsb, s);
-        if (nt == 4) return mw == 2 ? launch_bf<4, 2, true, false, 4>(a, ws, wsb, s) : launch_bf<4, 1, true, false, 4>(a, ws, wsb, s);
-        return mw == 2 ? launch_bf<1, 2, true, false, 4>(a, ws, wsb, s) : launch_bf<1, 1, true, false, 4>(a, ws, wsb, s);
+        if (nt == 8) {
+            if (env_int("BLADE_BF_NTSTORE", 1) == 0)
+                return mw == 2 ? launch_bf<8, 2, true, false, 4, false>(a, ws, wsb, s) : launch_bf<8, 1, true, false, 4, false>(a, ws, wsb, s);
+            return mw == 2 ? launch_bf<8, 2, true, false, 4, true>(a, ws, wsb, s) : launch_bf<8, 1, true, false, 4, true>(a, ws, wsb, s);
+        }
+        if (nt == 4) return mw == 2 ? launch_bf<4, 2, true, false, 4, true>(a, ws, wsb, s) : launch_bf<4, 1, true, false, 4, true>(a, ws, wsb, s);
+        return mw == 2 ? launch_bf<1, 2, true, false, 4, true>(a, ws, wsb, s) : launch_bf<1, 1, true, false, 4, true>(a, ws, wsb, s);
     } else {
-        if (nt == 4) return mw == 2 ? launch_bf<4, 2, false, false, 2>(a, ws, wsb, s) : launch_bf<4, 1, false, false, 2>(a, ws, wsb, s);
-        return mw == 2 ? launch_bf<1, 2, false, false, 2>(a, ws, wsb, s) : launch_bf<1, 1, false, false, 2>(a, ws, wsb, s);
+        if (nt == 4) return mw == 2 ? launch_bf<4, 2, false, false, 2, true>(a, ws, wsb, s) : launch_bf<4, 1, false, false, 2, true>(a, ws, wsb, s);
+        return mw == 2 ? launch_bf<1, 2, false, false, 2, true>(a, ws, wsb, s) : launch_bf<1, 1, false, false, 2, true>(a, ws, wsb, s);
     }
 }
 
@@ -685,7 +742,9 @@ static int run_beamformer(int in_type, int precision, const void* input, const v
     const int mw = B > 8 ? 2 : 1;
     if (precision == BLADE_PREC_F16X2) return dispatch<true, true>(a, T % 4 == 0 ? 4 : 1, mw, ws, wsb, s);
     if (in_type == BLADE_CI8) {
-        int nt = T % 8 == 0 ? 8 : (T % 4 == 0 ? 4 : 1);
+        // NT = 8 keeps two runs of 4 per lane: a wave then spans 128 samples,
+        // which must not straddle T.
+        int nt = T % 128 == 0 ? 8 : (T % 4 == 0 ? 4 : 1);
         const int want = env_int("BLADE_BF_NT", 0);
         if (want > 0 && want < nt && T % want == 0 && (rate == 1 || want % 4 == 0)) nt = want;
         return dispatch<true, false>(a, nt, mw, ws, wsb, s);
