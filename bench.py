@@ -187,7 +187,7 @@ def main():
         roof = {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 4)}
     roof["traffic"] = _traffic()
-    roof["kernel"] = "beamform_kernel<32,4,CI8,FP32> (fused beamform+FFT4+polarize)"
+    roof["kernel"] = "beamform_kernel<NT=4,MW=2,CI8,FP32,D=4,nt-stores> (fused beamform+FFT4+polarize) + bf_pack_kernel"
     roof["kernel_ms"] = round(f32["kernel_ms"], 5)
     roof["hbm_gbs"] = round(achieved_gbs, 1)
     roof["hbm_frac"] = round(achieved_gbs / HBM_PEAK_GBS, 4)

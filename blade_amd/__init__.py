@@ -14,6 +14,16 @@ import inspect as _inspect
 import sys as _sys
 from types import SimpleNamespace
 
+# One HIP runtime per process: PyTorch's libraries ask for the unversioned
+# ``libamdhip64.so`` (its bundled copy) while libblade_amd.so asks for the
+# SONAME ``libamdhip64.so.7``.  Loading torch first makes that SONAME resolve
+# to torch's copy; loading this library first would bring in a second runtime
+# that torch then duplicates.
+try:
+    import torch as _torch  # noqa: F401
+except ImportError:
+    _torch = None
+
 try:
     from . import _blade_ext as _ext
 except ImportError as _e:

@@ -35,6 +35,8 @@ SIGNATURES = {
     "blade_beamformer_workspace_size": [_i, _u64, _u64, _u64],
     "blade_beamform_chain_ws": [_i, _i, _p, _p, _p, _u64, _u64, _u64, _u64, _u64, _i, _i, _u64, _i,
                                 _p, _u64, _p],
+    "blade_beamformer_adjoint": [_p, _p, _p, _u64, _u64, _u64, _u64, _u64, _p],
+    "blade_beamformer_stokes_i_grad": [_p, _p, _p, _p, _p, _u64, _u64, _u64, _u64, _u64, _p],
     "blade_channelizer": [_p, _p, _u64, _u64, _u64, _u64, _u64, _p],
     "blade_channelizer_plan": [_u64, _p],
     "blade_channelizer_detector": [_p, _p, _u64, _u64, _u64, _u64, _u64, _u64, _i, _p],
@@ -68,6 +70,10 @@ def load(path: str | os.PathLike | None = None) -> ct.CDLL:
         raise RuntimeError(
             f"{p} is missing: build it with `python -m blade_amd.build` "
             "(the HIP library is required; there is no CPU fallback)")
+    try:
+        import torch  # noqa: F401  (one HIP runtime per process; see blade_amd/__init__.py)
+    except ImportError:
+        pass
     lib = ct.CDLL(str(p))
     for name, argtypes in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -1,0 +1,124 @@
+"""Differentiable beamforming for gain and delay calibration.
+
+Two ``torch.autograd.Function``s back onto the hand-written kernels of the
+C-ABI (``blade_beamformer_adjoint``, ``blade_beamformer_stokes_i_grad``):
+
+* :class:`Beamform` -- ``y = beamformer(x, w)`` (FP32 matrix cores); its
+  backward is the adjoint contraction ``grad_w = sum_t grad_y conj(x)``.
+* :class:`StokesIPower` -- ``L = -sum_b lambda_b sum |y_b|^2``, the negative
+  detected (Stokes-I) power of weighted beams.  Forward and gradient come from
+  one fused kernel that recomputes the beams tile by tile instead of storing
+  them over time; backward only scales the saved gradient.
+
+Complex gradients follow PyTorch's convention (``dL/dRe w + j dL/dIm w``,
+i.e. twice the conjugate Wirtinger derivative), so the functions compose with
+ordinary torch code for the parameterisation of the phasors.
+
+:func:`calibrate` runs Adam on per-antenna complex gains (unit modulus) and
+per-antenna delays that steer ``w_eff = w * g/|g| * exp(-j 2 pi f dtau)``.
+The forward model is the reference's beamformer
+(``src/modules/beamformer/beamformer.cu:13-79``); the calibration itself has
+no reference counterpart.
+"""
+
+from __future__ import annotations
+
+import math
+
+import torch
+
+from . import ops
+
+__all__ = ["Beamform", "StokesIPower", "beamform", "stokes_i_power", "calibrate",
+           "steer_phasors"]
+
+
+class Beamform(torch.autograd.Function):
+    """``y[b,f,t,p] = sum_a w[b,a,f,p] x[a,f,t,p]``; x CI8 (no gradient)."""
+
+    @staticmethod
+    def forward(ctx, x, w):
+        ctx.save_for_backward(x)
+        return ops.beamformer(x, w.contiguous(), precision="f32")
+
+    @staticmethod
+    def backward(ctx, grad_y):
+        (x,) = ctx.saved_tensors
+        grad_w = None
+        if ctx.needs_input_grad[1]:
+            grad_w = ops.beamformer_adjoint(x, grad_y.contiguous())
+        return None, grad_w
+
+
+class StokesIPower(torch.autograd.Function):
+    """``L = -sum_b beam_weights[b] sum_{f,t,p} |y_b|^2`` (FP64 scalar)."""
+
+    @staticmethod
+    def forward(ctx, x, w, beam_weights):
+        loss, grad = ops.stokes_i_loss_and_grad(x, w.contiguous(), beam_weights)
+        ctx.save_for_backward(grad)
+        return loss
+
+    @staticmethod
+    def backward(ctx, grad_loss):
+        (grad,) = ctx.saved_tensors
+        # PyTorch's complex gradient is twice dL/dconj(w).
+        return None, grad * (2.0 * grad_loss.to(torch.float32)), None
+
+
+def beamform(x: torch.Tensor, w: torch.Tensor) -> torch.Tensor:
+    return Beamform.apply(x, w)
+
+
+def stokes_i_power(x: torch.Tensor, w: torch.Tensor, beam_weights=None) -> torch.Tensor:
+    if beam_weights is None:
+        beam_weights = [1.0] + [0.0] * (w.shape[0] - 1)
+    return StokesIPower.apply(x, w, list(beam_weights))
+
+
+def steer_phasors(w: torch.Tensor, gains: torch.Tensor | None, delays: torch.Tensor | None,
+                  freqs_hz: torch.Tensor | None) -> torch.Tensor:
+    """``w_eff[b,a,f,0,p] = w * g/|g| [a,f,p] * exp(-j 2 pi freq[f] delays[a])``."""
+    out = w
+    if gains is not None:
+        unit = gains / gains.abs().clamp_min(1e-30)
+        out = out * unit[None, :, :, None, :]
+    if delays is not None:
+        phase = -2.0 * math.pi * delays[:, None] * freqs_hz[None, :]          # [A, F]
+        rot = torch.polar(torch.ones_like(phase), phase).to(w.dtype)
+        out = out * rot[None, :, :, None, None]
+    return out
+
+
+def calibrate(x: torch.Tensor, w: torch.Tensor, steps: int = 200, lr: float = 0.05,
+              beam_weights=None, fit_gains: bool = True, fit_delays: bool = False,
+              freqs_hz: torch.Tensor | None = None, delay_lr: float = 1e-10):
+    """Adam on unit-modulus gains [A,F,P] (and delays [A]) maximising beam power.
+
+    Returns ``(gains, delays, losses)``; the loss is normalised by its value at
+    the start so that its history is comparable across data sets.
+    """
+    A, F, P = w.shape[1], w.shape[2], w.shape[4]
+    params = []
+    gains = delays = None
+    if fit_gains:
+        gains = torch.ones((A, F, P), dtype=torch.complex64, device=w.device, requires_grad=True)
+        params.append({"params": [gains], "lr": lr})
+    if fit_delays:
+        if freqs_hz is None:
+            raise ValueError("delay calibration needs the channel frequencies")
+        delays = torch.zeros((A,), dtype=torch.float64, device=w.device, requires_grad=True)
+        params.append({"params": [delays], "lr": delay_lr})
+    opt = torch.optim.Adam(params)
+    losses = []
+    ref = None
+    for _ in range(steps):
+        opt.zero_grad(set_to_none=True)
+        w_eff = steer_phasors(w, gains, delays, freqs_hz)
+        loss = stokes_i_power(x, w_eff, beam_weights)
+        if ref is None:
+            ref = abs(float(loss.detach()))
+        (loss / ref).backward()
+        opt.step()
+        losses.append(float(loss.detach()) / ref)
+    return gains, delays, losses

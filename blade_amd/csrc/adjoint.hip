@@ -1,0 +1,330 @@
+// Adjoint of the ATA beamformer on CDNA4 matrix cores, and the fused
+// Stokes-I calibration gradient.
+//
+// For y[b,f,t,p] = sum_a w[b,a,f,p] x[a,f,t,p] (the beamformer of
+// reference src/modules/beamformer/beamformer.cu:13-79) and an upstream
+// gradient gy = dL/dconj(y), the phasor gradient is the contraction over time
+//     gw[b,a,f,p] = dL/dconj(w[b,a,f,p]) = sum_t gy[b,f,t,p] conj(x[a,f,t,p]).
+// Per (f, p) this is a (B x T)(T x A) product, embedded as the real GEMM
+//     [Re gw; Im gw] = [[gy_r, gy_i], [gy_i, -gy_r]] . [x_r; x_i]
+// with rows (beam, re/im), K = (time, component) and columns = antennas, on
+// v_mfma_f32_16x16x4f32 (one 16-row tile = 8 beams).
+//
+// Fused Stokes-I mode: L = -sum_b lambda_b sum_{f,t,p} |y_b|^2, so
+// gy = -lambda_b y_b.  Each workgroup recomputes its tile of y from the CI8
+// voltages already staged in LDS (the forward contraction, also on matrix
+// cores), turns it into gy in LDS and runs the adjoint on the same voltages:
+// the beams are never written to memory.
+//
+// Work split: blockIdx = (time split, f, beam tile of 8).  A workgroup walks
+// its time range in tiles of ADJ_TC samples (voltages of all antennas in LDS),
+// the four waves taking a quarter of each tile; partial gradients are summed
+// across waves in LDS and across time splits by adjoint_reduce_kernel, in a
+// fixed order (deterministic, no global atomics).
+#include "common.hh"
+
+#include <algorithm>
+#include <cstring>
+
+namespace bl {
+
+constexpr int ADJ_TC = 128;        // time samples per LDS tile
+constexpr int ADJ_WAVES = 4;
+constexpr int ADJ_MAXAT = 8;       // antenna tiles of 16: A <= 128
+constexpr int ADJ_MAXB = 64;       // beams of the fused loss
+
+struct AdjArgs {
+    const uint32_t* x;       // [A,F,T] dwords: CI8 (re0, im0, re1, im1)
+    const float2* w;         // [B,A,F,1,P] phasors (fused mode)
+    const float2* gy;        // [B,F,T,P] upstream gradient (adjoint mode)
+    float* partial;          // [nsplit][F][P][btiles][16][apad]
+    float* loss_part;        // [nsplit][F][btiles]
+    int A, F, T, B;
+    int apad, n_at, nsteps;  // padded antennas, antenna tiles, forward K steps
+    int tsplit;              // time samples per workgroup (multiple of ADJ_TC)
+    int btiles;
+    float lambda[ADJ_MAXB];  // beam weights of the fused loss
+};
+
+__device__ __forceinline__ floatx4 adj_mfma(float a, float b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ float ci8_component(uint32_t d, int idx) {
+    return (float)(int8_t)((d >> (8 * idx)) & 0xFFu);
+}
+
+template <bool FUSED>
+__global__ __launch_bounds__(256) void adjoint_kernel(AdjArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int apad = a.apad;
+    uint32_t* s_x = reinterpret_cast<uint32_t*>(lds_raw);              // [TC][apad]
+    float* s_gy = reinterpret_cast<float*>(s_x + ADJ_TC * apad);        // [TC][16]
+    float* s_G = s_gy + ADJ_TC * 16;                                    // [2][16][apad]
+    float* s_w = s_G + 2 * 16 * apad;                                   // [2][nsteps][64]
+    __shared__ float s_loss;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int i = lane & 15;
+    const int g = lane >> 4;
+    const int sp = blockIdx.x;
+    const int f = blockIdx.y;
+    const int bt = blockIdx.z;
+    const int b0 = bt * 8;
+    const int A = a.A, F = a.F, T = a.T, B = a.B;
+    const int nsteps = a.nsteps;
+    const int t_begin = sp * a.tsplit;
+    const int t_end = min(T, t_begin + a.tsplit);
+
+    for (int e = tid; e < 2 * 16 * apad; e += blockDim.x) s_G[e] = 0.f;
+    if (tid == 0) s_loss = 0.f;
+    if constexpr (FUSED) {
+        // Forward A operand of beams b0..b0+7, both pols:
+        // [p][step][g][row], step = 2 * antenna group + component.
+        for (int e = tid; e < 2 * nsteps * 64; e += blockDim.x) {
+            const int p = e / (nsteps * 64);
+            const int r = e - p * nsteps * 64;
+            const int st = r >> 6, kg = (r >> 4) & 3, row = r & 15;
+            const int an = (st >> 1) * 4 + kg, c = st & 1;
+            const int b = b0 + (row >> 1), co = row & 1;
+            float v = 0.f;
+            if (b < B && an < A) {
+                const float2 w = a.w[(((size_t)b * A + an) * F + f) * 2 + p];
+                v = (co == 0) ? (c == 0 ? w.x : -w.y) : (c == 0 ? w.y : w.x);
+            }
+            s_w[e] = v;
+        }
+    }
+
+    floatx4 acc2[2][ADJ_MAXAT];
+#pragma unroll
+    for (int p = 0; p < 2; p++)
+#pragma unroll
+        for (int at = 0; at < ADJ_MAXAT; at++) acc2[p][at] = floatx4{0.f, 0.f, 0.f, 0.f};
+    float loss = 0.f;
+
+    for (int t0 = t_begin; t0 < t_end; t0 += ADJ_TC) {
+        __syncthreads();   // the previous tile's readers are done
+        // Voltages of every antenna for this tile, s_x[t][a] (zero-padded).
+        for (int e = tid; e < ADJ_TC * apad; e += blockDim.x) {
+            const int an = e / ADJ_TC, t = e - an * ADJ_TC;
+            const int tt = t0 + t;
+            uint32_t v = 0u;
+            if (an < A && tt < t_end) v = a.x[((size_t)an * F + f) * T + tt];
+            s_x[t * apad + an] = v;
+        }
+        __syncthreads();
+
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            if constexpr (FUSED) {
+                // y tile: rows (beam, re/im), columns = time; each wave owns
+                // 32 samples (two 16-column tiles).
+                floatx4 acc[2] = {floatx4{0.f, 0.f, 0.f, 0.f}, floatx4{0.f, 0.f, 0.f, 0.f}};
+                for (int st = 0; st < nsteps; st++) {
+                    const float wv = s_w[(p * nsteps + st) * 64 + lane];
+                    const int an = (st >> 1) * 4 + g, c = st & 1;
+#pragma unroll
+                    for (int n = 0; n < 2; n++) {
+                        const int t = wave * 32 + n * 16 + i;
+                        const float xv = ci8_component(s_x[t * apad + an], 2 * p + c);
+                        acc[n] = adj_mfma(wv, xv, acc[n]);
+                    }
+                }
+                // lane (g, i), register r: row 4g + r = (beam 2g + r/2, comp r&1)
+#pragma unroll
+                for (int n = 0; n < 2; n++) {
+                    const int t = wave * 32 + n * 16 + i;
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int b = b0 + 2 * g + (r >> 1);
+                        const float lam = b < B ? a.lambda[b] : 0.f;
+                        loss += lam * acc[n][r] * acc[n][r];
+                        s_gy[t * 16 + 4 * g + r] = -lam * acc[n][r];
+                    }
+                }
+            } else {
+                for (int e = tid; e < ADJ_TC * 8; e += blockDim.x) {
+                    const int bb = e / ADJ_TC, t = e - bb * ADJ_TC;
+                    const int b = b0 + bb, tt = t0 + t;
+                    float2 v = make_float2(0.f, 0.f);
+                    if (b < B && tt < t_end) v = a.gy[(((size_t)b * F + f) * T + tt) * 2 + p];
+                    s_gy[t * 16 + 2 * bb] = v.x;
+                    s_gy[t * 16 + 2 * bb + 1] = v.y;
+                }
+            }
+            __syncthreads();
+
+            // Adjoint: K = (time, component), 4 samples per MFMA pair.
+#pragma unroll 2
+            for (int t4 = 0; t4 < 8; t4++) {
+                const int t = wave * 32 + t4 * 4 + g;
+                const float gr = s_gy[t * 16 + 2 * (i >> 1)];
+                const float gi = s_gy[t * 16 + 2 * (i >> 1) + 1];
+                const float a0 = (i & 1) == 0 ? gr : gi;     // multiplies x_r
+                const float a1 = (i & 1) == 0 ? gi : -gr;    // multiplies x_i
+#pragma unroll
+                for (int at = 0; at < ADJ_MAXAT; at++) {
+                    if (at < a.n_at) {
+                        const uint32_t d = s_x[t * apad + at * 16 + i];
+                        acc2[p][at] = adj_mfma(a0, ci8_component(d, 2 * p), acc2[p][at]);
+                        acc2[p][at] = adj_mfma(a1, ci8_component(d, 2 * p + 1), acc2[p][at]);
+                    }
+                }
+            }
+            __syncthreads();   // s_gy is rewritten by the next pol
+        }
+    }
+
+    // Sum the four waves' partial gradients in LDS, then write this
+    // workgroup's [2][16][apad] block.
+#pragma unroll
+    for (int p = 0; p < 2; p++)
+#pragma unroll
+        for (int at = 0; at < ADJ_MAXAT; at++) {
+            if (at < a.n_at) {
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    atomicAdd(&s_G[(p * 16 + 4 * g + r) * apad + at * 16 + i], acc2[p][at][r]);
+            }
+        }
+    if constexpr (FUSED) {
+        for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off, 64);
+        if (lane == 0) atomicAdd(&s_loss, loss);
+    }
+    __syncthreads();
+    float* out = a.partial + ((((size_t)sp * F + f) * 2) * a.btiles + bt) * 16 * apad;
+    const size_t pstride = (size_t)a.btiles * 16 * apad;   // between pols
+    for (int e = tid; e < 2 * 16 * apad; e += blockDim.x) {
+        const int p = e / (16 * apad), r = e - p * 16 * apad;
+        out[p * pstride + r] = s_G[e];
+    }
+    if constexpr (FUSED) {
+        if (tid == 0) a.loss_part[((size_t)sp * F + f) * a.btiles + bt] = s_loss;
+    }
+}
+
+// gw[b,a,f,p] = sum over time splits of the partial rows (b, re) and (b, im).
+__global__ __launch_bounds__(256) void adjoint_reduce_kernel(AdjArgs a, int nsplit, float2* gw) {
+    const size_t n = (size_t)a.B * a.A * a.F * 2;
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const int p = (int)(idx & 1);
+    size_t r = idx >> 1;
+    const int f = (int)(r % a.F);
+    r /= a.F;
+    const int an = (int)(r % a.A);
+    const int b = (int)(r / a.A);
+    const int bt = b >> 3, row = 2 * (b & 7);
+    float re = 0.f, im = 0.f;
+    for (int sp = 0; sp < nsplit; sp++) {
+        const float* blk = a.partial + ((((size_t)sp * a.F + f) * 2 + p) * a.btiles + bt) * 16 * a.apad;
+        re += blk[row * a.apad + an];
+        im += blk[(row + 1) * a.apad + an];
+    }
+    gw[idx] = make_float2(re, im);   // [B,A,F,1,P] order: ((b*A + a)*F + f)*P + p
+}
+
+// loss = -sum of the workgroup partials, accumulated in FP64 in a fixed order.
+__global__ __launch_bounds__(256) void adjoint_loss_kernel(const float* part, size_t n, double* loss) {
+    __shared__ double s[256];
+    double v = 0.0;
+    for (size_t e = threadIdx.x; e < n; e += blockDim.x) v += (double)part[e];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *loss = -s[0];
+}
+
+static int run_adjoint(bool fused, const void* input, const void* phasors, const void* grad_out,
+                       const float* beam_weights, double* loss, void* grad_phasors, uint64_t A,
+                       uint64_t F, uint64_t T, uint64_t P, uint64_t B, hipStream_t s) {
+    const char* who = fused ? "stokes-i gradient" : "beamformer adjoint";
+    if (P != 2) return fail(std::string(who) + ": number of polarizations must be 2");
+    if (A == 0 || B == 0) return fail(std::string(who) + ": need at least one antenna and one beam");
+    if (A > 16 * ADJ_MAXAT) return fail(std::string(who) + ": at most 128 antennas");
+    if (fused && B > ADJ_MAXB) return fail(std::string(who) + ": at most 64 beams");
+    if (B > 4096 || F > 65535 || T > (1u << 30)) return fail(std::string(who) + ": shape too large");
+    if (!input || !grad_phasors || (fused ? (!phasors || !loss || !beam_weights) : !grad_out))
+        return fail(std::string(who) + ": null pointer");
+    if ((uintptr_t)input % 4 || (uintptr_t)grad_phasors % 8 || (phasors && (uintptr_t)phasors % 16) ||
+        (grad_out && (uintptr_t)grad_out % 8))
+        return fail(std::string(who) + ": misaligned buffer");
+    AdjArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.x = static_cast<const uint32_t*>(input);
+    a.w = static_cast<const float2*>(phasors);
+    a.gy = static_cast<const float2*>(grad_out);
+    a.A = (int)A; a.F = (int)F; a.T = (int)T; a.B = (int)B;
+    a.n_at = (int)ceil_div(A, 16);
+    a.apad = 16 * a.n_at;
+    a.nsteps = 2 * (int)ceil_div(A, 4);
+    a.btiles = (int)ceil_div(B, 8);
+    if (fused)
+        for (uint64_t b = 0; b < B; b++) a.lambda[b] = beam_weights[b];
+    // Time splits: about 2048 workgroups, each a whole number of tiles.
+    const long tiles = (long)ceil_div(std::max<uint64_t>(T, 1), ADJ_TC);
+    const long per_split = (long)F * a.btiles;
+    long nsplit = std::max(1L, std::min(tiles, (2048 + per_split - 1) / per_split));
+    const long tiles_per = (tiles + nsplit - 1) / nsplit;
+    nsplit = (tiles + tiles_per - 1) / tiles_per;
+    a.tsplit = (int)(tiles_per * ADJ_TC);
+
+    const size_t part_floats = (size_t)nsplit * F * 2 * a.btiles * 16 * a.apad;
+    const size_t loss_floats = (size_t)nsplit * F * a.btiles;
+    unsigned char* ws = nullptr;
+    const size_t ws_bytes = (part_floats + loss_floats) * sizeof(float);
+    if (F == 0) return BLADE_SUCCESS;
+    int rc = library_workspace(WS_ADJOINT, ws_bytes, s, &ws);
+    if (rc) return rc;
+    a.partial = reinterpret_cast<float*>(ws);
+    a.loss_part = a.partial + part_floats;
+
+    const size_t lds = ((size_t)ADJ_TC * a.apad + ADJ_TC * 16 + 2 * 16 * a.apad +
+                        (fused ? 2 * (size_t)a.nsteps * 64 : 0)) * 4;
+    const void* fn = fused ? reinterpret_cast<const void*>(&adjoint_kernel<true>)
+                           : reinterpret_cast<const void*>(&adjoint_kernel<false>);
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) {
+            set_error(std::string(who) + ": LDS attribute: " + hipGetErrorString(e));
+            return BLADE_HIP_ERROR;
+        }
+    }
+    dim3 grid((unsigned)nsplit, (unsigned)F, (unsigned)a.btiles);
+    if (fused)
+        hipLaunchKernelGGL(adjoint_kernel<true>, grid, dim3(64 * ADJ_WAVES), lds, s, a);
+    else
+        hipLaunchKernelGGL(adjoint_kernel<false>, grid, dim3(64 * ADJ_WAVES), lds, s, a);
+    const size_t n = (size_t)B * A * F * 2;
+    hipLaunchKernelGGL(adjoint_reduce_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, a,
+                       (int)nsplit, static_cast<float2*>(grad_phasors));
+    if (fused)
+        hipLaunchKernelGGL(adjoint_loss_kernel, dim3(1), dim3(256), 0, s, a.loss_part, loss_floats, loss);
+    return check_launch(who);
+}
+
+}  // namespace bl
+
+using namespace bl;
+
+extern "C" int blade_beamformer_adjoint(const void* input, const void* grad_output,
+                                        void* grad_phasors, uint64_t A, uint64_t F, uint64_t T,
+                                        uint64_t P, uint64_t B, blade_stream_t stream) {
+    return run_adjoint(false, input, nullptr, grad_output, nullptr, nullptr, grad_phasors, A, F, T,
+                       P, B, reinterpret_cast<hipStream_t>(stream));
+}
+
+extern "C" int blade_beamformer_stokes_i_grad(const void* input, const void* phasors,
+                                              const float* beam_weights, double* loss,
+                                              void* grad_phasors, uint64_t A, uint64_t F,
+                                              uint64_t T, uint64_t P, uint64_t B,
+                                              blade_stream_t stream) {
+    return run_adjoint(true, input, phasors, nullptr, beam_weights, loss, grad_phasors, A, F, T, P,
+                       B, reinterpret_cast<hipStream_t>(stream));
+}

@@ -36,8 +36,6 @@
 
 #include <algorithm>
 #include <cstdlib>
-#include <map>
-#include <mutex>
 
 namespace bl {
 
@@ -601,37 +599,6 @@ static int nsteps_for(bool f16x2, int A) {
 
 static int groups_for(int B, int mw) { return (B + mw * (TILE / 2) - 1) / (mw * (TILE / 2)); }
 
-// Library-owned pack buffers, one per (device, stream) so that calls on
-// different streams never share one; grown on demand.
-static std::mutex g_ws_mutex;
-static std::map<std::pair<int, hipStream_t>, std::pair<unsigned char*, size_t>> g_ws;
-
-static int library_workspace(size_t bytes, hipStream_t s, unsigned char** out) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return fail("beamformer: hipGetDevice failed");
-    std::lock_guard<std::mutex> lock(g_ws_mutex);
-    auto& e = g_ws[{dev, s}];
-    if (e.second < bytes) {
-        if (e.first) {
-            // The old buffer may still be read by work queued on this stream.
-            if (hipStreamSynchronize(s) != hipSuccess || hipFree(e.first) != hipSuccess) {
-                set_error("beamformer: could not release the old workspace");
-                return BLADE_HIP_ERROR;
-            }
-            e = {nullptr, 0};
-        }
-        void* p = nullptr;
-        hipError_t err = hipMalloc(&p, bytes);
-        if (err != hipSuccess) {
-            set_error(std::string("beamformer: workspace allocation: ") + hipGetErrorString(err));
-            return BLADE_HIP_ERROR;
-        }
-        e = {static_cast<unsigned char*>(p), bytes};
-    }
-    *out = e.first;
-    return BLADE_SUCCESS;
-}
-
 template <int NT, int MW, bool CI8, bool F16X2, int D, bool NTS>
 static int launch_bf(const BfArgs& args, unsigned char* ws, size_t ws_bytes, hipStream_t s) {
     BfArgs a = args;
@@ -645,14 +612,15 @@ static int launch_bf(const BfArgs& args, unsigned char* ws, size_t ws_bytes, hip
         if (ws_bytes < need) return fail("beamformer: workspace too small");
         a.pack = ws;
     } else {
-        const int rc = library_workspace(need, s, &a.pack);
+        const int rc = library_workspace(WS_BEAMFORMER_PACK, need, s, &a.pack);
         if (rc) return rc;
     }
     a.n_tchunks = (a.T + span - 1) / span;
-    // Chunks per workgroup: enough workgroups for ~3 rounds of 2 per CU.
+    // Chunks per workgroup: about 768 workgroups (one round at 3 per CU);
+    // measured best for both variants at the production shape.
     int cpw = env_int("BLADE_BF_CHUNKS_PER_WG", 0);
     if (cpw <= 0) {
-        const long target = 256L * 2 * 3;
+        const long target = 256L * 3;
         const long per_chunk = (long)a.F * groups;
         cpw = (int)std::max(1L, ((long)a.n_tchunks * per_chunk + target - 1) / target);
         cpw = std::min(cpw, a.n_tchunks);
@@ -742,11 +710,13 @@ static int run_beamformer(int in_type, int precision, const void* input, const v
     const int mw = B > 8 ? 2 : 1;
     if (precision == BLADE_PREC_F16X2) return dispatch<true, true>(a, T % 4 == 0 ? 4 : 1, mw, ws, wsb, s);
     if (in_type == BLADE_CI8) {
-        // NT = 8 keeps two runs of 4 per lane: a wave then spans 128 samples,
-        // which must not straddle T.
-        int nt = T % 128 == 0 ? 8 : (T % 4 == 0 ? 4 : 1);
-        const int want = env_int("BLADE_BF_NT", 0);
-        if (want > 0 && want < nt && T % want == 0 && (rate == 1 || want % 4 == 0)) nt = want;
+        // NT = 4 (139 VGPRs, 3 waves per SIMD) hides the drain of each
+        // chunk's beam stores, which every later load wait includes (vmcnt
+        // counts stores too); NT = 8 (two runs of 4 per lane, a wave spanning
+        // 128 samples that must not straddle T) reuses phasors more but runs
+        // 2 waves per SIMD.  BLADE_BF_NT=8 selects it.
+        int nt = T % 4 == 0 ? 4 : 1;
+        if (env_int("BLADE_BF_NT", 0) == 8 && T % 128 == 0) nt = 8;
         return dispatch<true, false>(a, nt, mw, ws, wsb, s);
     }
     return dispatch<false, false>(a, T % 4 == 0 ? 4 : 1, mw, ws, wsb, s);

@@ -18,7 +18,8 @@ from . import _lib
 from ._lib import call
 
 __all__ = ["type_of", "stream_handle", "cast", "beamformer", "beamform_chain", "channelizer",
-           "polarizer", "detector", "gather", "permutation", "phasor_ata"]
+           "polarizer", "detector", "gather", "permutation", "phasor_ata", "beamformer_adjoint",
+           "stokes_i_loss_and_grad"]
 
 
 def stream_handle(stream=None):
@@ -112,6 +113,40 @@ def beamform_chain(x: torch.Tensor, phasors: torch.Tensor, incoherent=False, inc
              A, F, T, P, B, int(incoherent), int(incoherent_sqrt), rate, int(polarize),
              stream_handle(stream))
     return out
+
+
+def beamformer_adjoint(x: torch.Tensor, grad_output: torch.Tensor, out: torch.Tensor | None = None,
+                       stream=None) -> torch.Tensor:
+    """dL/dconj(w) [B,A,F,1,P] = sum_t grad_output[b,f,t,p] conj(x[a,f,t,p]); x CI8 [A,F,T,P]."""
+    A, F, T, P = _shape4(x, "ci8")
+    if type_of(x) != "ci8" or x.dtype != torch.int8:
+        raise ValueError("the adjoint takes CI8 voltages")
+    B = grad_output.shape[0]
+    if tuple(grad_output.shape) != (B, F, T, P) or grad_output.dtype != torch.complex64:
+        raise ValueError(f"grad_output must be CF32 [B,{F},{T},{P}], got {tuple(grad_output.shape)}")
+    if out is None:
+        out = torch.empty((B, A, F, 1, P), dtype=torch.complex64, device=x.device)
+    call("blade_beamformer_adjoint", _ptr(x), _ptr(grad_output), _ptr(out), A, F, T, P, B,
+         stream_handle(stream))
+    return out
+
+
+def stokes_i_loss_and_grad(x: torch.Tensor, phasors: torch.Tensor, beam_weights,
+                           stream=None) -> tuple[torch.Tensor, torch.Tensor]:
+    """L = -sum_b beam_weights[b] sum |y_b|^2 (FP64 device scalar) and dL/dconj(w) CF32."""
+    A, F, T, P = _shape4(x, "ci8")
+    B = phasors.shape[0]
+    if tuple(phasors.shape) != (B, A, F, 1, P) or phasors.dtype != torch.complex64:
+        raise ValueError(f"phasors must be CF32 [B,{A},{F},1,{P}], got {tuple(phasors.shape)}")
+    lam = [float(v) for v in beam_weights]
+    if len(lam) != B:
+        raise ValueError(f"need one beam weight per beam ({B}), got {len(lam)}")
+    weights = (ct.c_float * B)(*lam)
+    loss = torch.empty((), dtype=torch.float64, device=x.device)
+    grad = torch.empty((B, A, F, 1, P), dtype=torch.complex64, device=x.device)
+    call("blade_beamformer_stokes_i_grad", _ptr(x), _ptr(phasors), ct.cast(weights, ct.c_void_p),
+         _ptr(loss), _ptr(grad), A, F, T, P, B, stream_handle(stream))
+    return loss, grad
 
 
 def channelizer(x: torch.Tensor, rate: int, out: torch.Tensor | None = None, stream=None):

@@ -104,6 +104,33 @@ int blade_beamform_chain_ws(int in_type, int precision, const void* input, const
                             blade_stream_t stream);
 
 /*
+ * Adjoint of the beamformer (no reference counterpart: the reference has no
+ * gradients; the forward it differentiates is beamformer.cu:13-79).
+ * For y[b,f,t,p] = sum_a w[b,a,f,p] x[a,f,t,p] and grad_output = dL/dconj(y)
+ * CF32 [B,F,T,P], writes grad_phasors = dL/dconj(w) CF32 [B,A,F,1,P]:
+ *     grad_phasors[b,a,f,p] = sum_t grad_output[b,f,t,p] * conj(x[a,f,t,p]).
+ * input: CI8 [A,F,T,P]; P = 2, A <= 128.  This is also the gradient in
+ * PyTorch's convention when grad_output is PyTorch's gradient of y.
+ */
+int blade_beamformer_adjoint(const void* input, const void* grad_output, void* grad_phasors,
+                             uint64_t A, uint64_t F, uint64_t T, uint64_t P, uint64_t B,
+                             blade_stream_t stream);
+
+/*
+ * Loss and phasor gradient of the detected beam power used for gain/delay
+ * calibration (no reference counterpart):
+ *     L = -sum_b beam_weights[b] sum_{f,t,p} |y[b,f,t,p]|^2   (Stokes I)
+ * loss: one FP64 value in device memory; grad_phasors = dL/dconj(w) CF32
+ * [B,A,F,1,P] (PyTorch's gradient is twice this).  The beams are recomputed
+ * tile by tile and never stored.  beam_weights: host array of B floats,
+ * B <= 64; P = 2, A <= 128.
+ */
+int blade_beamformer_stokes_i_grad(const void* input, const void* phasors,
+                                   const float* beam_weights, double* loss, void* grad_phasors,
+                                   uint64_t A, uint64_t F, uint64_t T, uint64_t P, uint64_t B,
+                                   blade_stream_t stream);
+
+/*
  * Channelizer — fftshift(FFT_C) of every C-sample chunk along T.
  * Replaces Modules::Channelizer<CF32,CF32>::process (src/modules/channelizer/base.cc:122-146,
  * pre-kernel channelizer.cu:5-19 and cuFFT C2C forward) with the general-rate

@@ -25,6 +25,7 @@ __all__ = [
     "detector_stepped", "cast", "gather", "permutation",
     "mode_b", "mode_h", "beamform_chain_c64", "norm_rel_err", "max_rms_rel_err",
     "beamformed_stokes_i_loss", "stokes_i_loss_grad_gains",
+    "beamformer_adjoint", "stokes_i_loss_and_grad_phasors",
     "NUMPY_TYPES",
 ]
 
@@ -384,6 +385,36 @@ def beamformed_stokes_i_loss(x, gains, phasors, beam=0):
     g = np.asarray(gains, dtype=np.complex128)                       # [A, F, P]
     y = np.einsum("afp,aftp->ftp", g * w, x, optimize=True)
     return -float(np.sum(y.real ** 2 + y.imag ** 2))
+
+
+def beamformer_adjoint(x, grad_output):
+    """Adjoint of the beamformer with respect to the phasors.
+
+    For ``y[b,f,t,p] = sum_a w[b,a,f,p] x[a,f,t,p]`` (``beamformer_ata``) and
+    ``grad_output = dL/dconj(y)`` ``[B, F, T, P]``, returns
+    ``dL/dconj(w)[b,a,f,0,p] = sum_t grad_output[b,f,t,p] conj(x[a,f,t,p])``.
+
+    Pinned by the defining identity of the adjoint,
+    ``Re <gy, J dw> = Re <adj(gy), dw>`` for random ``dw`` (``J`` the
+    beamformer's linear map in ``w``), and by finite differences of the
+    Stokes-I loss, whose gradient is ``2 * adj(-y)``.
+    """
+    x = _as_complex(x)
+    gy = np.asarray(grad_output, dtype=np.complex128)
+    g = np.einsum("bftp,aftp->bafp", gy, np.conj(x), optimize=True)
+    return g[:, :, :, None, :]
+
+
+def stokes_i_loss_and_grad_phasors(x, phasors, beam_weights):
+    """``L = -sum_b lambda_b sum_{f,t,p} |y_b|^2`` and ``dL/dconj(w)``.
+
+    ``dL/dconj(y_b) = -lambda_b y_b``, so the phasor gradient is
+    ``beamformer_adjoint(x, -lambda * y)``.
+    """
+    lam = np.asarray(beam_weights, dtype=np.float64)
+    y = beamformer_ata(x, phasors)
+    loss = -float(np.sum(lam[:, None, None, None] * (y.real ** 2 + y.imag ** 2)))
+    return loss, beamformer_adjoint(x, -lam[:, None, None, None] * y)
 
 
 def stokes_i_loss_grad_gains(x, gains, phasors, beam=0):

@@ -286,6 +286,57 @@ def test_calibration_gradient_closed_form():
         assert np.isclose(grad[idx].real, dre, rtol=1e-6) and np.isclose(grad[idx].imag, dim, rtol=1e-6)
 
 
+def test_adjoint_identity():
+    # Re <gy, J dw> == Re <adj(gy), dw> for the beamformer's linear map J in w.
+    rng = np.random.default_rng(11)
+    x = O.ci8_gaussian((5, 3, 16, 2), rng=rng)
+    gy = crand(4, 3, 16, 2)
+    for _ in range(3):
+        dw = crand(4, 5, 3, 1, 2)
+        lhs = np.vdot(gy, O.beamformer_ata(x, dw)).real
+        rhs = np.vdot(O.beamformer_adjoint(x, gy), dw).real
+        assert np.isclose(lhs, rhs, rtol=1e-10)
+    # a dropped conjugate on x breaks the identity
+    bad = np.einsum("bftp,aftp->bafp", gy, O.ci8_to_complex(x))[:, :, :, None, :]
+    dw = crand(4, 5, 3, 1, 2)
+    assert not np.isclose(np.vdot(gy, O.beamformer_ata(x, dw)).real, np.vdot(bad, dw).real, rtol=1e-3)
+
+
+def test_stokes_i_phasor_gradient_finite_differences():
+    # PyTorch's gradient (dL/dRe w + j dL/dIm w) is 2 dL/dconj(w).
+    rng = np.random.default_rng(12)
+    x = O.ci8_gaussian((4, 2, 8, 2), rng=rng)
+    w = crand(3, 4, 2, 1, 2).astype(np.complex128)
+    lam = np.array([1.0, 0.0, 0.5])
+    loss, g = O.stokes_i_loss_and_grad_phasors(x, w, lam)
+    ref_loss = sum(-lam[b] * np.sum(np.abs(O.beamformer_ata(x, w)[b]) ** 2) for b in range(3))
+    assert np.isclose(loss, ref_loss, rtol=1e-12)
+    h = 1e-6
+    for idx in [(0, 0, 0, 0, 0), (2, 3, 1, 0, 1), (1, 2, 0, 0, 1)]:
+        e = np.zeros_like(w)
+        e[idx] = h
+        f = lambda v: O.stokes_i_loss_and_grad_phasors(x, v, lam)[0]
+        dre = (f(w + e) - f(w - e)) / (2 * h)
+        dim = (f(w + 1j * e) - f(w - 1j * e)) / (2 * h)
+        assert np.isclose(2 * g[idx].real, dre, rtol=1e-6, atol=1e-3)
+        assert np.isclose(2 * g[idx].imag, dim, rtol=1e-6, atol=1e-3)
+    # beam 1 has zero weight: zero gradient
+    assert np.all(g[1] == 0)
+
+
+def test_gain_gradient_is_adjoint_chain_rule():
+    # d/dconj(g) of L(g * w0) = conj(w0) * dL/dconj(w_eff): ties the gains
+    # gradient (pinned by finite differences above) to the phasor adjoint.
+    rng = np.random.default_rng(13)
+    x = O.ci8_gaussian((4, 2, 8, 2), rng=rng)
+    w = crand(1, 4, 2, 1, 2).astype(np.complex128)
+    gains = crand(4, 2, 2).astype(np.complex128)
+    weff = w * gains[None, :, :, None, :]
+    _, gw = O.stokes_i_loss_and_grad_phasors(x, weff, [1.0])
+    via_adjoint = 2 * np.conj(w[0, :, :, 0, :]) * gw[0, :, :, 0, :]
+    assert np.allclose(via_adjoint, O.stokes_i_loss_grad_gains(x, gains, w), rtol=1e-10)
+
+
 def test_cpu_port_matches_oracle():
     x8 = O.ci8_gaussian((5, 3, 32, 2), rng=3)
     w = crand(4, 5, 3, 1, 2)
