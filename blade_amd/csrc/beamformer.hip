@@ -83,6 +83,7 @@ struct BfArgs {
     unsigned char* pack;    // packed phasor slabs, one per (f, beam group)
     size_t slab_bytes;      // bytes per slab
     size_t wbytes;          // bytes of A operands at the head of a slab
+    float4* sink;           // 64 x 16 B: target of the stores of lanes with nothing to store
 };
 
 // Packs the phasors of one (f, beam group) into the A-operand layout the
@@ -229,24 +230,40 @@ __device__ __forceinline__ void store4(float4* p, float2 X, float2 Y) {
 
 // Epilogue: beam b, both pols, the NT samples of a lane whose first sample is
 // t0 (TimeMap); writes the plain beam, or spectra of `rate` samples
-// (fftshifted) into [B', F*C, T/C, P], optionally polarized.
-template <int NT, bool NTS>
+// (fftshifted) into [B', F*C, T/C, P], optionally polarized.  Every rate
+// issues NT stores per call whatever `ok` says (lanes with nothing to store
+// write to a.sink): the store count of an epilogue is the same on every path,
+// which keeps the wait counts of the following loads exact.
+// Epilogue variant: EPI = 0 reads rate / polarize / incoherent from the
+// arguments; otherwise EPI = rate | 8 * polarize | 16 * incoherent, fixed at
+// compile time (a straight-line epilogue).
+__device__ __forceinline__ constexpr int epi_code(int rate, bool pol, bool inc) {
+    return rate | (pol ? 8 : 0) | (inc ? 16 : 0);
+}
+template <int EPI> __device__ __forceinline__ int epi_rate(const BfArgs& a) { return EPI ? (EPI & 7) : a.rate; }
+template <int EPI> __device__ __forceinline__ bool epi_pol(const BfArgs& a) { return EPI ? (EPI & 8) != 0 : a.polarize != 0; }
+template <int EPI> __device__ __forceinline__ bool epi_inc(const BfArgs& a) { return EPI ? (EPI & 16) != 0 : a.incoherent != 0; }
+
+template <int NT, bool NTS, int EPI>
 __device__ __forceinline__ void store_beam(const BfArgs& a, int b, int f, int t0,
-                                           float2 (&v)[2][NT]) {
+                                           float2 (&v)[2][NT], bool ok) {
     typedef TimeMap<NT> M;
-    if (a.rate == 1) {
-        if (a.polarize) {
+    float4* const sink = a.sink + (threadIdx.x & 63);
+    const int rate = epi_rate<EPI>(a);
+    const bool pol = epi_pol<EPI>(a);
+    if (rate == 1) {
+        if (pol) {
 #pragma unroll
             for (int n = 0; n < NT; n++) polarize2(v[0][n], v[1][n]);
         }
         float4* dst = reinterpret_cast<float4*>(a.y) + ((size_t)b * a.F + f) * a.T + t0;
 #pragma unroll
-        for (int n = 0; n < NT; n++) store4<NTS>(dst + M::off(n), v[0][n], v[1][n]);
+        for (int n = 0; n < NT; n++) store4<NTS>(ok ? dst + M::off(n) : sink, v[0][n], v[1][n]);
         return;
     }
     if constexpr (NT % 4 == 0) {
         float4* base = reinterpret_cast<float4*>(a.y);
-        if (a.rate == 4) {
+        if (rate == 4) {
             const int S = a.T / 4;
 #pragma unroll
             for (int h = 0; h < NT / 4; h++) {
@@ -266,9 +283,9 @@ __device__ __forceinline__ void store_beam(const BfArgs& a, int b, int f, int t0
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     float2 X = o[0][k], Y = o[1][k];
-                    if (a.polarize) polarize2(X, Y);
+                    if (pol) polarize2(X, Y);
                     const int kp = (k + 2) & 3;   // fftshift
-                    store4<NTS>(base + (((size_t)b * a.F + f) * 4 + kp) * S + s, X, Y);
+                    store4<NTS>(ok ? base + (((size_t)b * a.F + f) * 4 + kp) * S + s : sink, X, Y);
                 }
             }
         } else {   // rate 2
@@ -280,8 +297,8 @@ __device__ __forceinline__ void store_beam(const BfArgs& a, int b, int f, int t0
                 for (int k = 0; k < 2; k++) {
                     float2 X = k == 0 ? cadd(v[0][2 * h], v[0][2 * h + 1]) : csub(v[0][2 * h], v[0][2 * h + 1]);
                     float2 Y = k == 0 ? cadd(v[1][2 * h], v[1][2 * h + 1]) : csub(v[1][2 * h], v[1][2 * h + 1]);
-                    if (a.polarize) polarize2(X, Y);
-                    store4<NTS>(base + (((size_t)b * a.F + f) * 2 + (k ^ 1)) * S + s, X, Y);
+                    if (pol) polarize2(X, Y);
+                    store4<NTS>(ok ? base + (((size_t)b * a.F + f) * 2 + (k ^ 1)) * S + s : sink, X, Y);
                 }
             }
         }
@@ -320,7 +337,7 @@ __device__ __forceinline__ void load_words(uint32_t* dst, const uint32_t* src) {
 // walks `chunks_per_wg` consecutive chunks of WAVES*TILE*NT samples with the
 // phasors staged once; voltage loads run D units ahead of the MFMAs through
 // a register ring that crosses chunk boundaries.
-template <int NT, int MW, bool CI8, bool F16X2, int D, bool NTS>
+template <int NT, int MW, bool CI8, bool F16X2, int D, bool NTS, int EPI>
 __global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
     constexpr int AK = F16X2 ? 4 * KG : KG;       // antennas per unit
     typedef Unit<NT, CI8, F16X2> U_t;
@@ -336,7 +353,7 @@ __global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
     const int f = blockIdx.y;
     const int b_wg = blockIdx.z * MW * (TILE / 2);
     const int nsteps = a.nsteps;
-    const bool do_inc = a.incoherent && blockIdx.z == 0;
+    const bool do_inc = epi_inc<EPI>(a) && blockIdx.z == 0;
 
     const int n_units = F16X2 ? nsteps : nsteps / 2;
     const int upad = ((n_units + D - 1) / D) * D;
@@ -525,41 +542,43 @@ __global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
         // of the chunk loop (they would pin dozens of registers).
         int fo = f;
         asm volatile("" : "+s"(fo));
-        if (t_ok) {
 #pragma unroll
-            for (int m = 0; m < MW; m++) {
+        for (int m = 0; m < MW; m++) {
 #pragma unroll
-                for (int r = 0; r < 4; r += 2) {
-                    const int b = b_wg + m * (TILE / 2) + (4 * g + r) / 2;
-                    if (b < a.B) {
-                        float2 v[2][NT];
-#pragma unroll
-                        for (int p = 0; p < 2; p++)
-#pragma unroll
-                            for (int n = 0; n < NT; n++)
-                                v[p][n] = make_float2(acc[m][p][n][r] * unscale[p], acc[m][p][n][r + 1] * unscale[p]);
-                        store_beam<NT, NTS>(a, b, fo, t0, v);
-                    }
-                }
-            }
-        }
-        if (do_inc) {
-#pragma unroll
-            for (int p = 0; p < 2; p++)
-#pragma unroll
-                for (int n = 0; n < NT; n++) {
-                    inc[p][n] += __shfl_xor(inc[p][n], 16, 64);
-                    inc[p][n] += __shfl_xor(inc[p][n], 32, 64);
-                }
-            if (t_ok && g == 0) {
+            for (int r = 0; r < 4; r += 2) {
+                const int b = b_wg + m * (TILE / 2) + (4 * g + r) / 2;
                 float2 v[2][NT];
 #pragma unroll
                 for (int p = 0; p < 2; p++)
 #pragma unroll
-                    for (int n = 0; n < NT; n++)
-                        v[p][n] = make_float2(a.inc_sqrt ? sqrtf(inc[p][n]) : inc[p][n], 0.f);
-                store_beam<NT, NTS>(a, a.B, fo, t0, v);
+                    for (int n = 0; n < NT; n++) {
+                        if constexpr (F16X2)
+                            v[p][n] = make_float2(acc[m][p][n][r] * unscale[p], acc[m][p][n][r + 1] * unscale[p]);
+                        else
+                            v[p][n] = make_float2(acc[m][p][n][r], acc[m][p][n][r + 1]);
+                    }
+                store_beam<NT, NTS, EPI>(a, min(b, a.B - 1), fo, t0, v, t_ok && b < a.B);
             }
+        }
+        // The incoherent beam's stores are issued by every workgroup (to the
+        // sink where there is none) with plain stores: the sink stays in L2.
+        if (epi_inc<EPI>(a)) {
+            if (do_inc) {
+#pragma unroll
+                for (int p = 0; p < 2; p++)
+#pragma unroll
+                    for (int n = 0; n < NT; n++) {
+                        inc[p][n] += __shfl_xor(inc[p][n], 16, 64);
+                        inc[p][n] += __shfl_xor(inc[p][n], 32, 64);
+                    }
+            }
+            float2 v[2][NT];
+#pragma unroll
+            for (int p = 0; p < 2; p++)
+#pragma unroll
+                for (int n = 0; n < NT; n++)
+                    v[p][n] = make_float2(a.inc_sqrt ? sqrtf(inc[p][n]) : inc[p][n], 0.f);
+            store_beam<NT, false, EPI>(a, a.B, fo, t0, v, do_inc && t_ok && g == 0);
         }
     };
 
@@ -567,16 +586,23 @@ __global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
     // A ring slot is consumed before it is reloaded, so every slot keeps its
     // registers across iterations (no copies, no drain of the loads in
     // flight at the loop latch).
-    int j = 0;   // global unit index (chunk-major)
-    for (int c = 0; c < nch; c++) {
-        for (int ub = 0; ub < upad; ub += D) {
+    auto group = [&](int ub, int j) {   // units ub..ub+D-1; j = global unit index
 #pragma unroll
-            for (int d = 0; d < D; d++) {
-                if (ub + d < n_units) compute(ring[d], ub + d);
-                load_unit(j + d + D, ring[d]);
-            }
-            j += D;
+        for (int d = 0; d < D; d++) {
+            if (ub + d < n_units) compute(ring[d], ub + d);
+            load_unit(j + d + D, ring[d]);
         }
+    };
+    // Chunk 0, then every later chunk with its first group peeled: the peeled
+    // group follows the previous epilogue's stores in straight-line code, so
+    // its waits count those stores (in flight after the awaited loads) instead
+    // of draining them, and the steady-state loop only sees its own loads.
+    for (int ub = 0; ub < upad; ub += D) group(ub, ub);
+    epilogue(0);
+    zero_acc();
+    for (int c = 1; c < nch; c++) {
+        group(0, c * upad);
+        for (int ub = D; ub < upad; ub += D) group(ub, c * upad + ub);
         epilogue(c);
         zero_acc();
     }
@@ -593,13 +619,15 @@ static void slab_sizes(bool f16x2, int mw, int nsteps, size_t* wbytes, size_t* s
     *slab = ((*wbytes + 258 * sizeof(float)) + 255) / 256 * 256;
 }
 
+constexpr size_t kSinkBytes = 64 * sizeof(float4);
+
 static int nsteps_for(bool f16x2, int A) {
     return f16x2 ? (A + 4 * KG - 1) / (4 * KG) : 2 * ((A + KG - 1) / KG);
 }
 
 static int groups_for(int B, int mw) { return (B + mw * (TILE / 2) - 1) / (mw * (TILE / 2)); }
 
-template <int NT, int MW, bool CI8, bool F16X2, int D, bool NTS>
+template <int NT, int MW, bool CI8, bool F16X2, int D, bool NTS, int EPI = 0>
 static int launch_bf(const BfArgs& args, unsigned char* ws, size_t ws_bytes, hipStream_t s) {
     BfArgs a = args;
     a.nsteps = nsteps_for(F16X2, a.A);
@@ -607,14 +635,16 @@ static int launch_bf(const BfArgs& args, unsigned char* ws, size_t ws_bytes, hip
     const size_t lds = a.wbytes;
     const int span = WAVES * TILE * NT;
     const int groups = groups_for(a.B, MW);
-    const size_t need = a.slab_bytes * (size_t)a.F * groups;
+    const size_t need = a.slab_bytes * (size_t)a.F * groups + kSinkBytes;
     if (ws) {
         if (ws_bytes < need) return fail("beamformer: workspace too small");
+        if ((uintptr_t)ws % 256) return fail("beamformer: workspace must be 256-byte aligned");
         a.pack = ws;
     } else {
         const int rc = library_workspace(WS_BEAMFORMER_PACK, need, s, &a.pack);
         if (rc) return rc;
     }
+    a.sink = reinterpret_cast<float4*>(a.pack + (need - kSinkBytes));
     a.n_tchunks = (a.T + span - 1) / span;
     // Chunks per workgroup: about 768 workgroups (one round at 3 per CU);
     // measured best for both variants at the production shape.
@@ -627,7 +657,7 @@ static int launch_bf(const BfArgs& args, unsigned char* ws, size_t ws_bytes, hip
     }
     a.chunks_per_wg = cpw;
     dim3 grid((unsigned)((a.n_tchunks + cpw - 1) / cpw), (unsigned)a.F, (unsigned)groups);
-    const void* fn = reinterpret_cast<const void*>(&beamform_kernel<NT, MW, CI8, F16X2, D, NTS>);
+    const void* fn = reinterpret_cast<const void*>(&beamform_kernel<NT, MW, CI8, F16X2, D, NTS, EPI>);
     if (lds > 150 * 1024) return fail("beamformer: too many antennas for the LDS phasor stage");
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -647,17 +677,30 @@ static int launch_bf(const BfArgs& args, unsigned char* ws, size_t ws_bytes, hip
     }
     hipLaunchKernelGGL((bf_pack_kernel<MW, F16X2>), dim3(2 * MW, (unsigned)a.F, (unsigned)groups), dim3(256),
                        pack_lds, s, a);
-    hipLaunchKernelGGL((beamform_kernel<NT, MW, CI8, F16X2, D, NTS>), grid, dim3(64 * WAVES), lds, s, a);
+    hipLaunchKernelGGL((beamform_kernel<NT, MW, CI8, F16X2, D, NTS, EPI>), grid, dim3(64 * WAVES), lds, s, a);
     return check_launch("beamformer");
 }
 
 // Variant table: (time samples per lane, beam tiles per wave) by precision.
 // Beams go out with nontemporal stores (they are not re-read by this kernel);
-// BLADE_BF_NTSTORE=0 selects plain stores for the FP32 CI8 variant.
+// BLADE_BF_NTSTORE=0 selects plain stores for the FP32 CI8 NT = 8 variant.
+// The production epilogues (FFT-4 + polarizer; plain beams with or without
+// the incoherent beam) of the two main variants are compiled as fixed
+// straight-line code: with every path issuing the same stores, the loads that
+// follow an epilogue wait for themselves only, not for the beam stores.
+constexpr int EPI_MODEB = 4 | 8;          // rate 4, polarize, no incoherent beam
+constexpr int EPI_PLAIN = 1;              // rate 1
+constexpr int EPI_PLAIN_INC = 1 | 16;     // rate 1, incoherent beam
+
 template <bool CI8, bool F16X2>
-static int dispatch(const BfArgs& a, int nt, int mw, unsigned char* ws, size_t wsb, hipStream_t s) {
+static int dispatch(const BfArgs& a, int nt, int mw, int epi, unsigned char* ws, size_t wsb, hipStream_t s) {
     if constexpr (F16X2) {
-        if (nt == 4) return mw == 2 ? launch_bf<4, 2, true, true, 2, true>(a, ws, wsb, s) : launch_bf<4, 1, true, true, 2, true>(a, ws, wsb, s);
+        if (nt == 4 && mw == 2) {
+            if (epi == EPI_MODEB) return launch_bf<4, 2, true, true, 2, true, EPI_MODEB>(a, ws, wsb, s);
+            if (epi == EPI_PLAIN) return launch_bf<4, 2, true, true, 2, true, EPI_PLAIN>(a, ws, wsb, s);
+            return launch_bf<4, 2, true, true, 2, true>(a, ws, wsb, s);
+        }
+        if (nt == 4) return launch_bf<4, 1, true, true, 2, true>(a, ws, wsb, s);
         return mw == 2 ? launch_bf<1, 2, true, true, 2, true>(a, ws, wsb, s) : launch_bf<1, 1, true, true, 2, true>(a, ws, wsb, s);
     } else if constexpr (CI8) {
         if (nt == 8) {
@@ -665,7 +708,13 @@ static int dispatch(const BfArgs& a, int nt, int mw, unsigned char* ws, size_t w
                 return mw == 2 ? launch_bf<8, 2, true, false, 4, false>(a, ws, wsb, s) : launch_bf<8, 1, true, false, 4, false>(a, ws, wsb, s);
             return mw == 2 ? launch_bf<8, 2, true, false, 4, true>(a, ws, wsb, s) : launch_bf<8, 1, true, false, 4, true>(a, ws, wsb, s);
         }
-        if (nt == 4) return mw == 2 ? launch_bf<4, 2, true, false, 4, true>(a, ws, wsb, s) : launch_bf<4, 1, true, false, 4, true>(a, ws, wsb, s);
+        if (nt == 4 && mw == 2) {
+            if (epi == EPI_MODEB) return launch_bf<4, 2, true, false, 4, true, EPI_MODEB>(a, ws, wsb, s);
+            if (epi == EPI_PLAIN) return launch_bf<4, 2, true, false, 4, true, EPI_PLAIN>(a, ws, wsb, s);
+            if (epi == EPI_PLAIN_INC) return launch_bf<4, 2, true, false, 4, true, EPI_PLAIN_INC>(a, ws, wsb, s);
+            return launch_bf<4, 2, true, false, 4, true>(a, ws, wsb, s);
+        }
+        if (nt == 4) return launch_bf<4, 1, true, false, 4, true>(a, ws, wsb, s);
         return mw == 2 ? launch_bf<1, 2, true, false, 4, true>(a, ws, wsb, s) : launch_bf<1, 1, true, false, 4, true>(a, ws, wsb, s);
     } else {
         if (nt == 4) return mw == 2 ? launch_bf<4, 2, false, false, 2, true>(a, ws, wsb, s) : launch_bf<4, 1, false, false, 2, true>(a, ws, wsb, s);
@@ -707,8 +756,10 @@ static int run_beamformer(int in_type, int precision, const void* input, const v
     a.pack = nullptr;
     a.slab_bytes = 0;
     a.wbytes = 0;
+    a.sink = nullptr;
     const int mw = B > 8 ? 2 : 1;
-    if (precision == BLADE_PREC_F16X2) return dispatch<true, true>(a, T % 4 == 0 ? 4 : 1, mw, ws, wsb, s);
+    const int epi = (int)rate | (polarize ? 8 : 0) | (incoherent ? 16 : 0);
+    if (precision == BLADE_PREC_F16X2) return dispatch<true, true>(a, T % 4 == 0 ? 4 : 1, mw, epi, ws, wsb, s);
     if (in_type == BLADE_CI8) {
         // NT = 4 (139 VGPRs, 3 waves per SIMD) hides the drain of each
         // chunk's beam stores, which every later load wait includes (vmcnt
@@ -717,9 +768,9 @@ static int run_beamformer(int in_type, int precision, const void* input, const v
         // 2 waves per SIMD.  BLADE_BF_NT=8 selects it.
         int nt = T % 4 == 0 ? 4 : 1;
         if (env_int("BLADE_BF_NT", 0) == 8 && T % 128 == 0) nt = 8;
-        return dispatch<true, false>(a, nt, mw, ws, wsb, s);
+        return dispatch<true, false>(a, nt, mw, epi, ws, wsb, s);
     }
-    return dispatch<false, false>(a, T % 4 == 0 ? 4 : 1, mw, ws, wsb, s);
+    return dispatch<false, false>(a, T % 4 == 0 ? 4 : 1, mw, epi, ws, wsb, s);
 }
 
 }  // namespace bl
@@ -750,7 +801,7 @@ extern "C" uint64_t blade_beamformer_workspace_size(int precision, uint64_t A, u
     const int mw = B > 8 ? 2 : 1;
     size_t wbytes = 0, slab = 0;
     slab_sizes(f16, mw, nsteps_for(f16, (int)A), &wbytes, &slab);
-    return (uint64_t)slab * F * groups_for((int)B, mw);
+    return (uint64_t)slab * F * groups_for((int)B, mw) + kSinkBytes;
 }
 
 extern "C" int blade_beamform_chain_ws(int in_type, int precision, const void* input,
