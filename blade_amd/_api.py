@@ -49,7 +49,7 @@ _device_lst = ["cpu", "cuda", "unified"]
 _types_lst = ["f16", "f32", "f64", "i8", "i16", "i32", "i64", "u8", "u16", "u32", "u64",
               "cf16", "cf32", "cf64", "ci8", "ci16", "ci32", "ci64", "cu8", "cu16", "cu32", "cu64"]
 _modules_lst = ["beamformer", "cast", "channelizer", "detector", "gather", "permutation",
-                "phasor", "polarizer", "modeh", "modeb"]
+                "phasor", "polarizer", "modeh", "modeb", "batchbeamformer"]
 
 for _n in _telescope_lst + _device_lst + _types_lst + _modules_lst:
     globals()[_n] = _Constant(_n)

@@ -35,6 +35,8 @@ SIGNATURES = {
     "blade_beamformer_workspace_size": [_i, _u64, _u64, _u64],
     "blade_beamform_chain_ws": [_i, _i, _p, _p, _p, _u64, _u64, _u64, _u64, _u64, _i, _i, _u64, _i,
                                 _p, _u64, _p],
+    "blade_beamform_batched_workspace_size": [_i, _p, _u64, _p],
+    "blade_beamform_batched": [_i, _p, _u64, _p, _p, _u64, _p],
     "blade_beamformer_adjoint": [_p, _p, _p, _u64, _u64, _u64, _u64, _u64, _p],
     "blade_beamformer_stokes_i_grad": [_p, _p, _p, _p, _p, _u64, _u64, _u64, _u64, _u64, _p],
     "blade_channelizer": [_p, _p, _u64, _u64, _u64, _u64, _u64, _p],
@@ -47,7 +49,20 @@ SIGNATURES = {
     "blade_phasor_ata": [_p, _p, _p, _i, _u64, _u64, _u64, _u64, _d, _d, _d, _u64, _p],
 }
 _RESTYPES = {"blade_amd_version": ct.c_char_p, "blade_amd_last_error": ct.c_char_p,
-             "blade_beamformer_workspace_size": ct.c_uint64}
+             "blade_beamformer_workspace_size": ct.c_uint64,
+             "blade_beamform_batched_workspace_size": ct.c_uint64}
+
+
+class BeamformBlock(ct.Structure):
+    """``blade_beamform_block`` of include/blade_amd.h."""
+    _fields_ = [("input", ct.c_void_p), ("phasors", ct.c_void_p), ("output", ct.c_void_p),
+                ("A", ct.c_uint64), ("F", ct.c_uint64), ("T", ct.c_uint64), ("B", ct.c_uint64)]
+
+
+class BatchTuning(ct.Structure):
+    """``blade_batch_tuning`` of include/blade_amd.h (0 = automatic)."""
+    _fields_ = [("time_samples_per_lane", ct.c_int32), ("beam_tiles", ct.c_int32),
+                ("chunks_per_item", ct.c_int32), ("workgroups_per_cu", ct.c_int32)]
 
 _lib = None
 

@@ -36,6 +36,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <cstring>
 
 namespace bl {
 
@@ -97,19 +98,17 @@ struct BfArgs {
 // (MW * 2, F, groups).  The split-FP16 scale is shared by the whole beam group
 // of a pol, so that path stages all MW * 8 beams of the pol for the maximum.
 template <int MW, bool F16X2>
-__global__ __launch_bounds__(256) void bf_pack_kernel(BfArgs a) {
+__device__ __forceinline__ void bf_pack_body(const BfArgs& a, const int mp, const int f, const int gz,
+                                             const int groups, float2* s_w, unsigned int& s_max) {
     constexpr int AK = F16X2 ? 4 * KG : KG;
     constexpr int NB = MW * (TILE / 2);              // beams of one group
-    constexpr int SB = F16X2 ? NB : TILE / 2;        // beams staged in LDS
-    extern __shared__ float2 s_w[];                  // [SB][A] phasors of pol p
-    __shared__ unsigned int s_max;
-    const int p = blockIdx.x & 1;
-    const int m = blockIdx.x >> 1;
-    const int f = blockIdx.y;
+    constexpr int SB = F16X2 ? NB : TILE / 2;        // beams staged in LDS (s_w[SB][A], pol p)
+    const int p = mp & 1;
+    const int m = mp >> 1;
     const int A = a.A;
     const int nsteps = a.nsteps;
-    const int b0 = blockIdx.z * NB + (F16X2 ? 0 : m * (TILE / 2));   // first staged beam
-    unsigned char* slab = a.pack + ((size_t)f * gridDim.z + blockIdx.z) * a.slab_bytes;
+    const int b0 = gz * NB + (F16X2 ? 0 : m * (TILE / 2));   // first staged beam
+    unsigned char* slab = a.pack + ((size_t)f * groups + gz) * a.slab_bytes;
     float* tail = reinterpret_cast<float*>(slab + a.wbytes);
     if (threadIdx.x == 0) s_max = 0u;
     float mx = 0.f;
@@ -179,7 +178,7 @@ __global__ __launch_bounds__(256) void bf_pack_kernel(BfArgs a) {
     if (m == 0) {
         // |w0|^2 of pol p (read only by the beam-group-0 workgroups, whose
         // staged row 0 is beam 0) and the unscale of pol p.
-        if (blockIdx.z == 0) {
+        if (gz == 0) {
             for (int an = threadIdx.x; an < 128; an += blockDim.x) {
                 float v = 0.f;
                 if (an < A) {
@@ -191,6 +190,13 @@ __global__ __launch_bounds__(256) void bf_pack_kernel(BfArgs a) {
         }
         if (threadIdx.x == 0) tail[256 + p] = unscale;
     }
+}
+
+template <int MW, bool F16X2>
+__global__ __launch_bounds__(256) void bf_pack_kernel(BfArgs a) {
+    extern __shared__ float2 s_wp[];
+    __shared__ unsigned int s_max;
+    bf_pack_body<MW, F16X2>(a, blockIdx.x, blockIdx.y, blockIdx.z, gridDim.z, s_wp, s_max);
 }
 
 __device__ __forceinline__ void polarize2(float2& X, float2& Y) {
@@ -337,27 +343,26 @@ __device__ __forceinline__ void load_words(uint32_t* dst, const uint32_t* src) {
 // walks `chunks_per_wg` consecutive chunks of WAVES*TILE*NT samples with the
 // phasors staged once; voltage loads run D units ahead of the MFMAs through
 // a register ring that crosses chunk boundaries.
+// One work item: channel f, beam group gz (of `groups`), chunks cw0 ..
+// cw0 + chunks_per_wg - 1 of the block described by `a`.  LDS: lds_raw holds
+// the phasor A operands, s_unscale / s_w0sq the slab's tail.
 template <int NT, int MW, bool CI8, bool F16X2, int D, bool NTS, int EPI>
-__global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
+__device__ __forceinline__ void bf_body(const BfArgs& a, const int f, const int gz, const int cw0,
+                                        const int groups, unsigned char* lds_raw,
+                                        float* s_unscale, float (*s_w0sq)[128]) {
     constexpr int AK = F16X2 ? 4 * KG : KG;       // antennas per unit
     typedef Unit<NT, CI8, F16X2> U_t;
-
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    __shared__ float s_unscale[2];
-    __shared__ float s_w0sq[2][128];
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int i = lane % TILE;     // A-operand row / accumulator column
     const int g = lane / TILE;     // k-group
-    const int f = blockIdx.y;
-    const int b_wg = blockIdx.z * MW * (TILE / 2);
+    const int b_wg = gz * MW * (TILE / 2);
     const int nsteps = a.nsteps;
-    const bool do_inc = epi_inc<EPI>(a) && blockIdx.z == 0;
+    const bool do_inc = epi_inc<EPI>(a) && gz == 0;
 
     const int n_units = F16X2 ? nsteps : nsteps / 2;
     const int upad = ((n_units + D - 1) / D) * D;
-    const int cw0 = blockIdx.x * a.chunks_per_wg;
     const int nch = min(a.chunks_per_wg, a.n_tchunks - cw0);
     const int J = nch * upad;
     const size_t ant_stride = (size_t)a.F * a.T;   // complex samples between antennas
@@ -409,7 +414,7 @@ __global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
     // ---- Copy this (f, beam group)'s packed phasors into LDS. ---------------
     // The slab was laid out by bf_pack_kernel: A operands, |w0|^2 per
     // antenna and pol, and the power-of-two unscale of the split-FP16 path.
-    const unsigned char* slab = a.pack + ((size_t)f * gridDim.z + blockIdx.z) * a.slab_bytes;
+    const unsigned char* slab = a.pack + ((size_t)f * groups + gz) * a.slab_bytes;
     {
         const uint4* src = reinterpret_cast<const uint4*>(slab);
         uint4* dst = reinterpret_cast<uint4*>(lds_raw);
@@ -608,6 +613,16 @@ __global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
     }
 }
 
+// One block: blockIdx = (chunk range, f, beam group).
+template <int NT, int MW, bool CI8, bool F16X2, int D, bool NTS, int EPI>
+__global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ float s_unscale[2];
+    __shared__ float s_w0sq[2][128];
+    bf_body<NT, MW, CI8, F16X2, D, NTS, EPI>(a, blockIdx.y, blockIdx.z, blockIdx.x * a.chunks_per_wg,
+                                            gridDim.z, lds_raw, s_unscale, s_w0sq);
+}
+
 static int env_int(const char* name, int dflt) {
     const char* v = getenv(name);
     return v ? atoi(v) : dflt;
@@ -722,6 +737,225 @@ static int dispatch(const BfArgs& a, int nt, int mw, int epi, unsigned char* ws,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Batched blocks of uneven shapes in one persistent launch.
+//
+// Every block (its own A, F, T, B and buffers) is cut into work items of
+// (chunk range, f, beam group); a fixed grid of workgroups walks the
+// concatenated item list with a static stride, so every workgroup exits after
+// its last item.  The block table travels in the kernel arguments (no copy,
+// capturable in a hipGraph).  Rate-1 beams, no incoherent beam.
+// ---------------------------------------------------------------------------
+constexpr int kBatchMax = 32;
+
+struct BatchBlock {
+    const void* x;
+    const float2* w;
+    float2* y;
+    unsigned char* pack;    // this block's phasor slabs
+    size_t slab_bytes, wbytes;
+    int A, F, T, B;
+    int nsteps, n_tchunks, cpw, groups, mw;
+    int item_base;          // first work item of the block
+    int pack_base;          // first pack workgroup of the block
+};
+
+struct BatchArgs {
+    int n_blocks, total_items, total_pack;
+    float4* sink;
+    BatchBlock blk[kBatchMax];
+};
+
+__device__ __forceinline__ BfArgs batch_block_args(const BatchArgs& ba, int k) {
+    const BatchBlock& b = ba.blk[k];
+    BfArgs a;
+    a.x = b.x; a.w = b.w; a.y = b.y;
+    a.A = b.A; a.F = b.F; a.T = b.T; a.B = b.B;
+    a.incoherent = 0; a.inc_sqrt = 0; a.rate = 1; a.polarize = 0;
+    a.nsteps = b.nsteps; a.n_tchunks = b.n_tchunks; a.chunks_per_wg = b.cpw;
+    a.pack = b.pack; a.slab_bytes = b.slab_bytes; a.wbytes = b.wbytes;
+    a.sink = ba.sink;
+    return a;
+}
+
+template <bool F16X2>
+__global__ __launch_bounds__(256) void bf_pack_batched_kernel(BatchArgs ba) {
+    extern __shared__ float2 s_wp[];
+    __shared__ unsigned int s_max;
+    const int id = blockIdx.x;
+    int k = 0;
+    while (k + 1 < ba.n_blocks && id >= ba.blk[k + 1].pack_base) k++;
+    const BatchBlock& b = ba.blk[k];
+    int local = id - b.pack_base;
+    const int mp = local % (2 * b.mw);
+    local /= 2 * b.mw;
+    const int f = local % b.F;
+    const int gz = local / b.F;
+    const BfArgs a = batch_block_args(ba, k);
+    if (b.mw == 2)
+        bf_pack_body<2, F16X2>(a, mp, f, gz, b.groups, s_wp, s_max);
+    else
+        bf_pack_body<1, F16X2>(a, mp, f, gz, b.groups, s_wp, s_max);
+}
+
+template <int NT, bool F16X2, int D>
+__global__ __launch_bounds__(256, 2) void beamform_batched_kernel(BatchArgs ba) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ float s_unscale[2];
+    __shared__ float s_w0sq[2][128];
+    for (int item = blockIdx.x; item < ba.total_items; item += gridDim.x) {
+        int k = 0;
+        while (k + 1 < ba.n_blocks && item >= ba.blk[k + 1].item_base) k++;
+        const BatchBlock& b = ba.blk[k];
+        int local = item - b.item_base;
+        const int gz = local % b.groups;
+        local /= b.groups;
+        const int f = local % b.F;
+        const int tr = local / b.F;
+        const BfArgs a = batch_block_args(ba, k);
+        __syncthreads();   // the previous item's readers of LDS are done
+        if (b.mw == 2)
+            bf_body<NT, 2, true, F16X2, D, true, EPI_PLAIN>(a, f, gz, tr * b.cpw, b.groups, lds_raw,
+                                                            s_unscale, s_w0sq);
+        else
+            bf_body<NT, 1, true, F16X2, D, true, EPI_PLAIN>(a, f, gz, tr * b.cpw, b.groups, lds_raw,
+                                                            s_unscale, s_w0sq);
+    }
+}
+
+struct BatchPlan {
+    BatchArgs args;
+    size_t ws_bytes;        // slabs + sink
+    size_t lds, pack_lds;
+    int grid;
+};
+
+static int device_cus() {
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cached[dev]) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cached[dev] = n;
+    }
+    return cached[dev];
+}
+
+// Plans up to kBatchMax blocks; the workspace pointer may be null (sizing).
+static int plan_batch(bool f16x2, const blade_beamform_block* blocks, int n,
+                      const blade_batch_tuning* tuning, unsigned char* ws, BatchPlan* plan) {
+    constexpr int NT = 4;
+    const int span = WAVES * TILE * NT;
+    const int mw_force = tuning ? tuning->beam_tiles : 0;
+    const int cpw_req = tuning ? tuning->chunks_per_item : 0;
+    const int wpc = (tuning && tuning->workgroups_per_cu > 0) ? tuning->workgroups_per_cu : 3;
+    if (tuning && tuning->time_samples_per_lane != 0 && tuning->time_samples_per_lane != NT)
+        return fail("batched beamformer: time_samples_per_lane must be 0 or 4");
+    if (mw_force < 0 || mw_force > 2) return fail("batched beamformer: beam_tiles must be 0, 1 or 2");
+    BatchArgs& ba = plan->args;
+    std::memset(&ba, 0, sizeof(ba));
+    ba.n_blocks = n;
+    size_t off = 0, lds = 0, pack_lds = 0;
+    long items = 0, packs = 0;
+    // Auto chunk count: about wpc * CUs * 2 items in total (two rounds).
+    long all_chunks = 0;
+    for (int k = 0; k < n; k++) {
+        const blade_beamform_block& bk = blocks[k];
+        const int mw = mw_force ? mw_force : (bk.B > 8 ? 2 : 1);
+        all_chunks += (long)ceil_div(bk.T, span) * bk.F * groups_for((int)bk.B, mw);
+    }
+    const long target = 2L * wpc * device_cus();
+    const int cpw_auto = (int)std::max(1L, (all_chunks + target - 1) / target);
+    for (int k = 0; k < n; k++) {
+        const blade_beamform_block& bk = blocks[k];
+        if (bk.A == 0 || bk.B == 0 || bk.F == 0 || bk.T == 0)
+            return fail("batched beamformer: every block needs A, F, T, B > 0");
+        if (bk.A > 4096 || bk.F > 65535 || bk.T > (1u << 30) || bk.B > 4096)
+            return fail("batched beamformer: block shape too large");
+        if (bk.T % NT) return fail("batched beamformer: T must be a multiple of 4");
+        if (!bk.input || !bk.phasors || !bk.output) return fail("batched beamformer: null pointer");
+        if ((uintptr_t)bk.input % 16 || (uintptr_t)bk.phasors % 16 || (uintptr_t)bk.output % 16)
+            return fail("batched beamformer: buffers must be 16-byte aligned");
+        BatchBlock& b = ba.blk[k];
+        b.mw = mw_force ? mw_force : (bk.B > 8 ? 2 : 1);
+        b.x = bk.input;
+        b.w = static_cast<const float2*>(bk.phasors);
+        b.y = static_cast<float2*>(bk.output);
+        b.A = (int)bk.A; b.F = (int)bk.F; b.T = (int)bk.T; b.B = (int)bk.B;
+        b.nsteps = nsteps_for(f16x2, b.A);
+        slab_sizes(f16x2, b.mw, b.nsteps, &b.wbytes, &b.slab_bytes);
+        b.groups = groups_for(b.B, b.mw);
+        b.n_tchunks = (int)ceil_div(bk.T, span);
+        b.cpw = std::min(b.n_tchunks, cpw_req > 0 ? cpw_req : cpw_auto);
+        b.pack = ws ? ws + off : nullptr;
+        off += ((b.slab_bytes * (size_t)b.F * b.groups) + 255) / 256 * 256;
+        b.item_base = (int)items;
+        b.pack_base = (int)packs;
+        items += (long)ceil_div(b.n_tchunks, b.cpw) * b.F * b.groups;
+        packs += (long)2 * b.mw * b.F * b.groups;
+        lds = std::max(lds, b.wbytes);
+        pack_lds = std::max(pack_lds, (size_t)(f16x2 ? b.mw : 1) * (TILE / 2) * b.A * sizeof(float2));
+    }
+    if (items > (1L << 30) || packs > (1L << 30)) return fail("batched beamformer: too many work items");
+    if (lds > 150 * 1024 || pack_lds > 150 * 1024) return fail("batched beamformer: too many antennas");
+    ba.total_items = (int)items;
+    ba.total_pack = (int)packs;
+    ba.sink = ws ? reinterpret_cast<float4*>(ws + off) : nullptr;
+    plan->ws_bytes = off + kSinkBytes;
+    plan->lds = lds;
+    plan->pack_lds = pack_lds;
+    plan->grid = (int)std::min<long>(items, (long)wpc * device_cus());
+    return BLADE_SUCCESS;
+}
+
+template <bool F16X2>
+static int launch_batch(BatchPlan& plan, hipStream_t s) {
+    constexpr int D = F16X2 ? 2 : 4;
+    const void* fn = reinterpret_cast<const void*>(&beamform_batched_kernel<4, F16X2, D>);
+    const void* pfn = reinterpret_cast<const void*>(&bf_pack_batched_kernel<F16X2>);
+    if (plan.lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)plan.lds);
+        if (e != hipSuccess) return fail(std::string("batched beamformer: LDS attribute: ") + hipGetErrorString(e));
+    }
+    if (plan.pack_lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(pfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)plan.pack_lds);
+        if (e != hipSuccess) return fail(std::string("batched beamformer: pack LDS attribute: ") + hipGetErrorString(e));
+    }
+    hipLaunchKernelGGL(bf_pack_batched_kernel<F16X2>, dim3((unsigned)plan.args.total_pack), dim3(256),
+                       plan.pack_lds, s, plan.args);
+    hipLaunchKernelGGL((beamform_batched_kernel<4, F16X2, D>), dim3((unsigned)plan.grid), dim3(64 * WAVES),
+                       plan.lds, s, plan.args);
+    return check_launch("batched beamformer");
+}
+
+static int run_batched(int precision, const blade_beamform_block* blocks, uint64_t n,
+                       const blade_batch_tuning* tuning, unsigned char* ws, size_t ws_bytes,
+                       uint64_t* size_out, hipStream_t s) {
+    if (precision != BLADE_PREC_F32 && precision != BLADE_PREC_F16X2)
+        return fail("batched beamformer: unknown precision");
+    if (n > 0 && !blocks) return fail("batched beamformer: null block table");
+    const bool f16 = precision == BLADE_PREC_F16X2;
+    size_t total = 0;
+    for (uint64_t k0 = 0; k0 < n; k0 += kBatchMax) {
+        const int cnt = (int)std::min<uint64_t>(kBatchMax, n - k0);
+        BatchPlan plan;
+        unsigned char* base = ws ? ws + total : nullptr;
+        if (base && (uintptr_t)base % 256) return fail("batched beamformer: workspace must be 256-byte aligned");
+        int rc = plan_batch(f16, blocks + k0, cnt, tuning, base, &plan);
+        if (rc) return rc;
+        const size_t need = (plan.ws_bytes + 255) / 256 * 256;
+        if (!size_out) {
+            if (!ws || ws_bytes < total + need) return fail("batched beamformer: workspace too small");
+            rc = f16 ? launch_batch<true>(plan, s) : launch_batch<false>(plan, s);
+            if (rc) return rc;
+        }
+        total += need;
+    }
+    if (size_out) *size_out = total;
+    return BLADE_SUCCESS;
+}
+
 static int run_beamformer(int in_type, int precision, const void* input, const void* phasors,
                           void* output, uint64_t A, uint64_t F, uint64_t T, uint64_t P,
                           uint64_t B, int incoherent, int incoherent_sqrt, uint64_t rate,
@@ -814,4 +1048,21 @@ extern "C" int blade_beamform_chain_ws(int in_type, int precision, const void* i
     return run_beamformer(in_type, precision, input, phasors, output, A, F, T, P, B, incoherent,
                           incoherent_sqrt, rate, polarize, static_cast<unsigned char*>(workspace),
                           (size_t)workspace_bytes, reinterpret_cast<hipStream_t>(stream));
+}
+
+extern "C" uint64_t blade_beamform_batched_workspace_size(int precision,
+                                                         const blade_beamform_block* blocks,
+                                                         uint64_t n_blocks,
+                                                         const blade_batch_tuning* tuning) {
+    uint64_t size = 0;
+    if (run_batched(precision, blocks, n_blocks, tuning, nullptr, 0, &size, nullptr)) return 0;
+    return size;
+}
+
+extern "C" int blade_beamform_batched(int precision, const blade_beamform_block* blocks,
+                                      uint64_t n_blocks, const blade_batch_tuning* tuning,
+                                      void* workspace, uint64_t workspace_bytes,
+                                      blade_stream_t stream) {
+    return run_batched(precision, blocks, n_blocks, tuning, static_cast<unsigned char*>(workspace),
+                       (size_t)workspace_bytes, nullptr, reinterpret_cast<hipStream_t>(stream));
 }

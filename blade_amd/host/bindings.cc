@@ -176,6 +176,30 @@ void bind_beamformer(py::module_& m, const char* in_name, const char* out_name) 
     tm.attr("input") = gen.attr("input");
 }
 
+void bind_batched_beamformer(py::module_& m) {
+    using M = Modules::Beamformer::Batched<CI8, CF32>;
+    auto mm = m.def_submodule("in_ci8").def_submodule("out_cf32");
+    py::class_<M, Module, std::shared_ptr<M>> cls(mm, "mod");
+    py::class_<typename M::Config>(mm, "config")
+        .def(py::init([](BOOL split, int nt, int bt, int cpi, int wpc) {
+                 return typename M::Config{split, blade_batch_tuning{nt, bt, cpi, wpc}};
+             }),
+             "enable_split_fp16"_a = false, "time_samples_per_lane"_a = 0, "beam_tiles"_a = 0,
+             "chunks_per_item"_a = 0, "workgroups_per_cu"_a = 0);
+    py::class_<typename M::Input>(mm, "input")
+        .def(py::init([](const std::vector<ArrayTensor<Device::CUDA, CI8>>& bufs,
+                         const std::vector<PhasorTensor<Device::CUDA, CF32>>& phasors) {
+                 return typename M::Input{bufs, phasors};
+             }),
+             "buffers"_a, "phasors"_a);
+    cls.def(py::init<const typename M::Config&, const typename M::Input&, const Stream&>(), "config"_a,
+            "input"_a, "stream"_a)
+        .def("process", [](M& m, U64 counter) { return m.process(counter); }, "counter"_a = 0)
+        .def("get_config", &M::getConfig, py::return_value_policy::reference_internal)
+        .def("get_output", [](M& m) { return m.getOutputBuffers(); })
+        .def("__repr__", [](M&) { return std::string("BatchedBeamformer()"); });
+}
+
 template <typename IT, typename OT>
 void bind_channelizer(py::module_& m, const char* in_name, const char* out_name) {
     using M = Modules::Channelizer<IT, OT>;
@@ -519,6 +543,9 @@ PYBIND11_MODULE(_blade_ext, m) {
     auto bf = m.def_submodule("_beamformer_impl");
     bind_beamformer<CF32, CF32>(bf, "in_cf32", "out_cf32");
     bind_beamformer<CI8, CF32>(bf, "in_ci8", "out_cf32");
+
+    auto bb = m.def_submodule("_batchbeamformer_impl");
+    bind_batched_beamformer(bb);
 
     auto ch = m.def_submodule("_channelizer_impl");
     bind_channelizer<CF32, CF32>(ch, "in_cf32", "out_cf32");

@@ -103,6 +103,52 @@ template class BLADE_API Generic<CI8, CF32>;
 template class BLADE_API ATA<CF32, CF32>;
 template class BLADE_API ATA<CI8, CF32>;
 
+template <typename IT, typename OT>
+Batched<IT, OT>::Batched(const Config& config, const Input& input, const Stream&)
+    : config(config), input(input) {
+    static_assert(std::is_same<IT, CI8>::value, "the batched beamformer takes CI8 voltages");
+    if (input.bufs.size() != input.phasors.size()) {
+        BL_FATAL("Batched beamformer: {} input buffers but {} phasor tensors.", input.bufs.size(),
+                 input.phasors.size());
+        throw Result::ERROR;
+    }
+    for (U64 k = 0; k < input.bufs.size(); k++) {
+        const auto& in = input.bufs[k].shape();
+        const auto& ph = input.phasors[k].shape();
+        if (ph.numberOfAntennas() != in.numberOfAspects() ||
+            ph.numberOfFrequencyChannels() != in.numberOfFrequencyChannels() ||
+            ph.numberOfPolarizations() != in.numberOfPolarizations() || in.numberOfPolarizations() != 2) {
+            BL_FATAL("Batched beamformer: block {} has input {} and phasors {}.", k, in, ph);
+            throw Result::ERROR;
+        }
+        output.bufs.emplace_back(ArrayShape({ph.numberOfBeams(), in.numberOfFrequencyChannels(),
+                                             in.numberOfTimeSamples(), 2}));
+        table.push_back(blade_beamform_block{input.bufs[k].data(), input.phasors[k].data(),
+                                             output.bufs[k].data(), in.numberOfAspects(),
+                                             in.numberOfFrequencyChannels(), in.numberOfTimeSamples(),
+                                             ph.numberOfBeams()});
+    }
+    const int prec = config.enableSplitFp16 ? BLADE_PREC_F16X2 : BLADE_PREC_F32;
+    const U64 bytes = blade_beamform_batched_workspace_size(prec, table.data(), table.size(), &config.tuning);
+    if (bytes == 0 && !table.empty()) {
+        BL_FATAL("Batched beamformer: {}", blade_amd_last_error());
+        throw Result::ERROR;
+    }
+    workspace = Vector<Device::CUDA, U8, VectorShape>({std::max<U64>(bytes, 256)});
+    BL_INFO("Blocks: {}, workspace {} bytes, arithmetic {}", table.size(), bytes,
+            config.enableSplitFp16 ? "split FP16 MFMA" : "FP32 MFMA");
+}
+
+template <typename IT, typename OT>
+Result Batched<IT, OT>::process(const U64&, const Stream& stream) {
+    const int prec = config.enableSplitFp16 ? BLADE_PREC_F16X2 : BLADE_PREC_F32;
+    BL_ABI_CHECK(blade_beamform_batched(prec, table.data(), table.size(), &config.tuning,
+                                        workspace.data(), workspace.size(), abi(stream)));
+    return Result::SUCCESS;
+}
+
+template class BLADE_API Batched<CI8, CF32>;
+
 }  // namespace Modules::Beamformer
 
 // --------------------------------------------------------------- Channelizer

@@ -19,7 +19,7 @@ from ._lib import call
 
 __all__ = ["type_of", "stream_handle", "cast", "beamformer", "beamform_chain", "channelizer",
            "polarizer", "detector", "gather", "permutation", "phasor_ata", "beamformer_adjoint",
-           "stokes_i_loss_and_grad"]
+           "stokes_i_loss_and_grad", "BatchedBeamformer"]
 
 
 def stream_handle(stream=None):
@@ -113,6 +113,55 @@ def beamform_chain(x: torch.Tensor, phasors: torch.Tensor, incoherent=False, inc
              A, F, T, P, B, int(incoherent), int(incoherent_sqrt), rate, int(polarize),
              stream_handle(stream))
     return out
+
+
+class BatchedBeamformer:
+    """Beamformer blocks of different shapes in one persistent launch.
+
+    ``blocks``: list of ``(x CI8 [A,F,T,P,2], phasors CF32 [B,A,F,1,P],
+    out CF32 [B,F,T,P])`` device tensors (kept referenced).  The block table
+    and the workspace are built once; every call launches the batch on
+    ``stream`` and allocates nothing (capturable in a hipGraph).
+    ``tuning``: dict of the ``blade_batch_tuning`` fields (missing = auto).
+    """
+
+    def __init__(self, blocks, precision="f32", tuning=None):
+        self.blocks = [tuple(b) for b in blocks]
+        self.precision = {"f32": _lib.PREC_F32, "f16x2": _lib.PREC_F16X2}[precision]
+        self.table = (_lib.BeamformBlock * max(1, len(self.blocks)))()
+        for k, (x, w, y) in enumerate(self.blocks):
+            A, F, T, P = _shape4(x, "ci8")
+            B = w.shape[0]
+            if type_of(x) != "ci8":
+                raise ValueError("batched blocks take CI8 voltages")
+            if tuple(w.shape) != (B, A, F, 1, P) or w.dtype != torch.complex64:
+                raise ValueError(f"block {k}: phasors must be CF32 [B,{A},{F},1,{P}]")
+            if tuple(y.shape) != (B, F, T, P) or y.dtype != torch.complex64:
+                raise ValueError(f"block {k}: output must be CF32 [{B},{F},{T},{P}]")
+            if P != 2:
+                raise ValueError("two polarizations")
+            self.table[k] = _lib.BeamformBlock(_ptr(x).value, _ptr(w).value, _ptr(y).value, A, F, T, B)
+        self.set_tuning(tuning)
+
+    def set_tuning(self, tuning=None):
+        t = dict(tuning or {})
+        self.tuning = _lib.BatchTuning(int(t.get("time_samples_per_lane", 0)), int(t.get("beam_tiles", 0)),
+                                       int(t.get("chunks_per_item", 0)), int(t.get("workgroups_per_cu", 0)))
+        lib = _lib.load()
+        n = len(self.blocks)
+        size = lib.blade_beamform_batched_workspace_size(self.precision, self.table, n, ct.byref(self.tuning))
+        if size == 0 and n:
+            raise _lib.BladeError("blade_beamform_batched_workspace_size", 1,
+                                  lib.blade_amd_last_error().decode())
+        dev = self.blocks[0][0].device if n else torch.device("cuda")
+        if getattr(self, "workspace", None) is None or self.workspace.numel() < size:
+            self.workspace = torch.empty(max(size, 256), dtype=torch.uint8, device=dev)
+        return self
+
+    def __call__(self, stream=None):
+        call("blade_beamform_batched", self.precision, self.table, len(self.blocks), ct.byref(self.tuning),
+             _ptr(self.workspace), self.workspace.numel(), stream_handle(stream))
+        return [y for (_, _, y) in self.blocks]
 
 
 def beamformer_adjoint(x: torch.Tensor, grad_output: torch.Tensor, out: torch.Tensor | None = None,

@@ -2,6 +2,7 @@
 #define BLADE_MODULES_BASE_HH
 
 #include "blade/modules/beamformer/ata.hh"
+#include "blade/modules/beamformer/batched.hh"
 #include "blade/modules/cast.hh"
 #include "blade/modules/channelizer/base.hh"
 #include "blade/modules/detector.hh"

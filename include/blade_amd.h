@@ -104,6 +104,43 @@ int blade_beamform_chain_ws(int in_type, int precision, const void* input, const
                             blade_stream_t stream);
 
 /*
+ * Many beamformer blocks of different shapes in one persistent launch (no
+ * reference counterpart; each block is what one Beamformer::ATA module call
+ * computes, src/modules/beamformer/beamformer.cu:13-79, without the
+ * incoherent beam).  Each block: CI8 input [A,F,T,2], CF32 phasors
+ * [B,A,F,1,2], CF32 output [B,F,T,2]; T a multiple of 4.  The call is
+ * asynchronous on `stream`, allocates nothing, and can be captured in a
+ * hipGraph: size the caller-owned workspace with
+ * blade_beamform_batched_workspace_size (same blocks, same tuning).
+ */
+typedef struct blade_beamform_block {
+    const void* input;
+    const void* phasors;
+    void* output;
+    uint64_t A, F, T, B;
+} blade_beamform_block;
+
+/*
+ * Tile shape and launch shape of the batched launch; 0 = automatic.
+ *   time_samples_per_lane: 4
+ *   beam_tiles: 16-row beam tiles per wavefront, 1 (8 beams) or 2 (16)
+ *   chunks_per_item: time chunks of 256 samples per work item
+ *   workgroups_per_cu: resident workgroups per compute unit (grid size)
+ */
+typedef struct blade_batch_tuning {
+    int32_t time_samples_per_lane;
+    int32_t beam_tiles;
+    int32_t chunks_per_item;
+    int32_t workgroups_per_cu;
+} blade_batch_tuning;
+
+uint64_t blade_beamform_batched_workspace_size(int precision, const blade_beamform_block* blocks,
+                                               uint64_t n_blocks, const blade_batch_tuning* tuning);
+int blade_beamform_batched(int precision, const blade_beamform_block* blocks, uint64_t n_blocks,
+                           const blade_batch_tuning* tuning, void* workspace,
+                           uint64_t workspace_bytes, blade_stream_t stream);
+
+/*
  * Adjoint of the beamformer (no reference counterpart: the reference has no
  * gradients; the forward it differentiates is beamformer.cu:13-79).
  * For y[b,f,t,p] = sum_a w[b,a,f,p] x[a,f,t,p] and grad_output = dL/dconj(y)
