@@ -11,6 +11,11 @@ value = beamformed Msamples/s = B*F*T*P / (time per block) / 1e6.
 The ``cpu_baseline`` leg times the vectorised numpy port of the same chain
 on the host's cores and checks both GPU variants against the complex128
 oracle on a few channels.
+
+Extra legs (not the headline value; ``--no-extra`` skips them):
+``batched`` -- 32 uneven blocks in one persistent launch, autotuned;
+``mode_a`` -- split-FP16 beamform -> FFT-8192 -> Stokes I on A28 F192 T16384 B8;
+``calibration`` -- fused Stokes-I loss/gradient and Adam steps on A42 F384 T2048 B4.
 """
 
 from __future__ import annotations
@@ -147,6 +152,118 @@ def cpu_leg(x, w, outs, budget_s=12.0):
     }, errs
 
 
+def _events_ms(fn, iters):
+    import torch
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(iters):
+        fn()
+    e1.record(s)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / iters
+
+
+def _ci8(shape, gen):
+    import torch
+    v = torch.randn(tuple(shape) + (2,), device="cuda", generator=gen) * 16.0
+    return torch.clamp(torch.round(v), -127, 127).to(torch.int8)
+
+
+def batched_leg(iters=20):
+    """32 uneven blocks (A in {14,20,27,28}, F in {64..1024}, T in {1024..8192},
+    B in 4..64, drawn from seed 2026) in one persistent launch, autotuned."""
+    import torch
+    from blade_amd import autotune, ops
+    rng = np.random.default_rng(2026)
+    shapes = [(int(rng.choice([14, 20, 27, 28])), int(rng.choice([64, 128, 256, 1024])),
+               int(rng.choice([1024, 2048, 4096, 8192])), int(rng.integers(4, 65))) for _ in range(32)]
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    blocks = []
+    for (a, f, t, b) in shapes:
+        x = _ci8((a, f, t, 2), gen)
+        ph = torch.polar(torch.ones((b, a, f, 1, 2), device="cuda"),
+                         torch.rand((b, a, f, 1, 2), device="cuda", generator=gen) * 6.283185307179586)
+        blocks.append((x, ph.to(torch.complex64), torch.empty((b, f, t, 2), dtype=torch.complex64, device="cuda")))
+    n_out = sum(b * f * t * 2 for (a, f, t, b) in shapes)
+    bytes_ = sum(2 * a * f * t * 2 + 8 * b * f * t * 2 + 8 * b * a * f * 2 for (a, f, t, b) in shapes)
+    flops = sum(8 * a * b * f * t * 2 for (a, f, t, b) in shapes)
+    out = {"workload": "32 blocks, seed 2026", "blocks": len(shapes), "beamformed_samples": n_out,
+           "alg_bytes": bytes_, "alg_flops": flops}
+    for prec in ("f32", "f16x2"):
+        tuning, _ = autotune.tune_batched(blocks, prec, iters=3, warmup=2)
+        bb = ops.BatchedBeamformer(blocks, prec, tuning)
+        _events_ms(bb, 5)
+        ms = _events_ms(bb, iters)
+        out[prec] = {"ms_per_batch": round(ms, 4), "msamples_per_s": round(n_out / ms / 1e3, 1),
+                     "hbm_gbs": round(bytes_ / ms / 1e6, 1), "tflops": round(flops / ms / 1e9, 2),
+                     "tuning": tuning}
+    return out
+
+
+def mode_a_leg(iters=20):
+    """Mixed-precision Mode A block: CI8 A28 F192 T16384, split-FP16 beamform of
+    B8 -> FFT-8192 -> Stokes I over 4 spectra (2 blocks), F32 filterbank."""
+    import torch
+    from blade_amd import ops
+    A_, F_, T_, B_, C, N = 28, 192, 16384, 8, 8192, 4
+    gen = torch.Generator(device="cuda").manual_seed(4)
+    x = _ci8((A_, F_, T_, 2), gen)
+    rng = np.random.default_rng(2)
+    tau = torch.from_numpy(rng.uniform(-200e-9, 200e-9, size=(B_, A_))).cuda()
+    cal = torch.ones((A_, F_, 1, 2), dtype=torch.complex128, device="cuda")
+    w = ops.phasor_ata(tau, cal, 9.904e9, 0.5e6, 0.0, 0, out_type="cf32")
+    y = torch.empty((B_, F_, T_, 2), dtype=torch.complex64, device="cuda")
+    out = torch.empty((B_, F_ * C, 1, 1), dtype=torch.float32, device="cuda")
+    state = {"k": 0}
+
+    def block():
+        ops.beamformer(x, w, False, False, "f16x2", out=y)
+        ops.channelizer_detector(y, C, N, 1, out=out, accumulate=state["k"] % 2 == 1)
+        state["k"] += 1
+
+    _events_ms(block, 10)
+    ms = _events_ms(block, iters)
+    t_bf = _events_ms(lambda: ops.beamformer(x, w, False, False, "f16x2", out=y), iters)
+    t_cd = _events_ms(lambda: ops.channelizer_detector(y, C, N, 1, out=out, accumulate=True), iters)
+    return {"workload": "CI8 A28 F192 T16384 -> split-FP16 beamform B8 -> FFT-8192 -> Stokes I (4 spectra, 2 blocks)",
+            "ms_per_block": round(ms, 4), "beamform_ms": round(t_bf, 4), "channelize_detect_ms": round(t_cd, 4),
+            "msamples_per_s": round(B_ * F_ * T_ * 2 / ms / 1e3, 1),
+            "realtime_factor": round((T_ / 0.5e6) / (ms * 1e-3), 1)}
+
+
+def calibration_leg(steps=200):
+    """Gain calibration on A42 F384 T2048 B4: fused Stokes-I loss + gradient
+    (beams recomputed, never stored) and full Adam steps on unit-modulus gains."""
+    import torch
+    from blade_amd import ops
+    from blade_amd.calibration import calibrate
+    A_, F_, T_, B_ = 42, 384, 2048, 4
+    gen = torch.Generator(device="cuda").manual_seed(6)
+    x = _ci8((A_, F_, T_, 2), gen)
+    w = torch.polar(torch.ones((B_, A_, F_, 1, 2), device="cuda"),
+                    torch.rand((B_, A_, F_, 1, 2), device="cuda", generator=gen) * 6.283185307179586).to(torch.complex64)
+    lam = [1.0, 0.0, 0.0, 0.0]
+    _events_ms(lambda: ops.stokes_i_loss_and_grad(x, w, lam), 5)
+    g_ms = _events_ms(lambda: ops.stokes_i_loss_and_grad(x, w, lam), 20)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, _, losses = calibrate(x, w, steps=steps, lr=0.05)
+    torch.cuda.synchronize()
+    step_ms = (time.perf_counter() - t0) * 1e3 / steps
+    bytes_ = A_ * F_ * T_ * 2 * 2
+    return {"workload": "A42 F384 T2048 P2 B4, loss = -Stokes I of beam 0", "grad_ms": round(g_ms, 4),
+            "grad_input_gbs": round(bytes_ / g_ms / 1e6, 1), "adam_step_ms": round(step_ms, 3),
+            "adam_steps": steps, "loss_first_last": [round(losses[0], 4), round(losses[-1], 4)]}
+
+
+def _extra(fn):
+    try:
+        return fn()
+    except Exception as e:  # an extra leg never hides the headline line
+        return {"error": f"{type(e).__name__}: {e}"}
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as fh:
@@ -164,6 +281,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the batched, Mode-A and calibration legs")
     ap.add_argument("--kernel-only", choices=["f32", "f16x2"],
                     help="profiling mode: launch only this variant of the fused kernel, then a "
                          "plain 1 GiB device copy (blade_gather) that calibrates HBM counters")
@@ -225,6 +344,10 @@ def main():
         "split_f16": split,
         "realtime_factor": round((T / 0.5e6) / t_step, 1),
     }
+    if not args.no_extra:
+        line["batched"] = _extra(batched_leg)
+        line["mode_a"] = _extra(mode_a_leg)
+        line["calibration"] = _extra(calibration_leg)
     if not args.no_cpu:
         x, w, outs = bufs
         cpu, errs = cpu_leg(x, w, outs)

@@ -206,12 +206,15 @@ __device__ __forceinline__ void polarize2(float2& X, float2& Y) {
     Y = R;
 }
 
-// Time sample of a lane's n-th column: lanes own runs of RUN = min(NT, 4)
-// consecutive samples, the runs of one lane TILE * RUN samples apart, so a
-// 16-lane group reads and writes TILE * RUN contiguous samples per access.
-template <int NT>
+// Time sample of a lane's n-th column: lanes own runs of RUN consecutive
+// samples, the runs of one lane TILE * RUN samples apart, so a 16-lane group
+// reads and writes TILE * RUN contiguous samples per access.  RUN = 4 gives
+// 16-byte voltage loads and keeps the 4 samples of a DFT-4 in one lane; the
+// plain (rate 1) epilogues use RUN = 1, where each store instruction writes
+// 16 lanes x 16 contiguous bytes instead of 16 bytes of every 64.
+template <int NT, int RUN_ = (NT < 4 ? NT : 4)>
 struct TimeMap {
-    static constexpr int RUN = NT < 4 ? NT : 4;
+    static constexpr int RUN = RUN_;
     // offset of column n from the lane's first sample (chunk base + RUN * i)
     static __device__ __forceinline__ constexpr int off(int n) {
         return (n / RUN) * (TILE * RUN) + n % RUN;
@@ -247,13 +250,14 @@ __device__ __forceinline__ constexpr int epi_code(int rate, bool pol, bool inc) 
     return rate | (pol ? 8 : 0) | (inc ? 16 : 0);
 }
 template <int EPI> __device__ __forceinline__ int epi_rate(const BfArgs& a) { return EPI ? (EPI & 7) : a.rate; }
+template <int NT, int EPI> constexpr int epi_run() { return (EPI && (EPI & 7) == 1) ? 1 : (NT < 4 ? NT : 4); }
 template <int EPI> __device__ __forceinline__ bool epi_pol(const BfArgs& a) { return EPI ? (EPI & 8) != 0 : a.polarize != 0; }
 template <int EPI> __device__ __forceinline__ bool epi_inc(const BfArgs& a) { return EPI ? (EPI & 16) != 0 : a.incoherent != 0; }
 
 template <int NT, bool NTS, int EPI>
 __device__ __forceinline__ void store_beam(const BfArgs& a, int b, int f, int t0,
                                            float2 (&v)[2][NT], bool ok) {
-    typedef TimeMap<NT> M;
+    typedef TimeMap<NT, epi_run<NT, EPI>()> M;
     float4* const sink = a.sink + (threadIdx.x & 63);
     const int rate = epi_rate<EPI>(a);
     const bool pol = epi_pol<EPI>(a);
@@ -367,7 +371,7 @@ __device__ __forceinline__ void bf_body(const BfArgs& a, const int f, const int 
     const int J = nch * upad;
     const size_t ant_stride = (size_t)a.F * a.T;   // complex samples between antennas
 
-    typedef TimeMap<NT> M;
+    typedef TimeMap<NT, epi_run<NT, EPI>()> M;
     // First sample of this lane in chunk c (runs of M::RUN samples, TimeMap).
     auto chunk_t0 = [&](int c) { return ((cw0 + c) * WAVES + wave) * (TILE * NT) + M::RUN * i; };
 
@@ -697,44 +701,41 @@ static int launch_bf(const BfArgs& args, unsigned char* ws, size_t ws_bytes, hip
 }
 
 // Variant table: (time samples per lane, beam tiles per wave) by precision.
-// Beams go out with nontemporal stores (they are not re-read by this kernel);
-// BLADE_BF_NTSTORE=0 selects plain stores for the FP32 CI8 NT = 8 variant.
 // The production epilogues (FFT-4 + polarizer; plain beams with or without
-// the incoherent beam) of the two main variants are compiled as fixed
+// the incoherent beam) of the NT = 4 variants are compiled as fixed
 // straight-line code: with every path issuing the same stores, the loads that
 // follow an epilogue wait for themselves only, not for the beam stores.
+// Stores: nontemporal where the 16 lanes of a group write contiguous 16-byte
+// records (the fixed epilogues); in the generic epilogue a store instruction
+// writes 16 bytes of every 32 or 64 and ordinary stores let L2 merge the lines
+// before they go out.
 constexpr int EPI_MODEB = 4 | 8;          // rate 4, polarize, no incoherent beam
 constexpr int EPI_PLAIN = 1;              // rate 1
 constexpr int EPI_PLAIN_INC = 1 | 16;     // rate 1, incoherent beam
 
+template <int NT, int MW, bool CI8, bool F16X2, int D>
+static int launch_epi(const BfArgs& a, int epi, unsigned char* ws, size_t wsb, hipStream_t s) {
+    if constexpr (NT == 4) {
+        if (epi == EPI_MODEB) return launch_bf<NT, MW, CI8, F16X2, D, true, EPI_MODEB>(a, ws, wsb, s);
+        // The rate-1 epilogues map a wave onto TILE * NT consecutive samples
+        // (runs of one): T must not cut a wave.
+        if (a.T % (TILE * NT) == 0) {
+            if (epi == EPI_PLAIN) return launch_bf<NT, MW, CI8, F16X2, D, true, EPI_PLAIN>(a, ws, wsb, s);
+            if (epi == EPI_PLAIN_INC)
+                return launch_bf<NT, MW, CI8, F16X2, D, true, EPI_PLAIN_INC>(a, ws, wsb, s);
+        }
+    }
+    return launch_bf<NT, MW, CI8, F16X2, D, false, 0>(a, ws, wsb, s);
+}
+
 template <bool CI8, bool F16X2>
 static int dispatch(const BfArgs& a, int nt, int mw, int epi, unsigned char* ws, size_t wsb, hipStream_t s) {
-    if constexpr (F16X2) {
-        if (nt == 4 && mw == 2) {
-            if (epi == EPI_MODEB) return launch_bf<4, 2, true, true, 2, true, EPI_MODEB>(a, ws, wsb, s);
-            if (epi == EPI_PLAIN) return launch_bf<4, 2, true, true, 2, true, EPI_PLAIN>(a, ws, wsb, s);
-            return launch_bf<4, 2, true, true, 2, true>(a, ws, wsb, s);
-        }
-        if (nt == 4) return launch_bf<4, 1, true, true, 2, true>(a, ws, wsb, s);
-        return mw == 2 ? launch_bf<1, 2, true, true, 2, true>(a, ws, wsb, s) : launch_bf<1, 1, true, true, 2, true>(a, ws, wsb, s);
-    } else if constexpr (CI8) {
-        if (nt == 8) {
-            if (env_int("BLADE_BF_NTSTORE", 1) == 0)
-                return mw == 2 ? launch_bf<8, 2, true, false, 4, false>(a, ws, wsb, s) : launch_bf<8, 1, true, false, 4, false>(a, ws, wsb, s);
-            return mw == 2 ? launch_bf<8, 2, true, false, 4, true>(a, ws, wsb, s) : launch_bf<8, 1, true, false, 4, true>(a, ws, wsb, s);
-        }
-        if (nt == 4 && mw == 2) {
-            if (epi == EPI_MODEB) return launch_bf<4, 2, true, false, 4, true, EPI_MODEB>(a, ws, wsb, s);
-            if (epi == EPI_PLAIN) return launch_bf<4, 2, true, false, 4, true, EPI_PLAIN>(a, ws, wsb, s);
-            if (epi == EPI_PLAIN_INC) return launch_bf<4, 2, true, false, 4, true, EPI_PLAIN_INC>(a, ws, wsb, s);
-            return launch_bf<4, 2, true, false, 4, true>(a, ws, wsb, s);
-        }
-        if (nt == 4) return launch_bf<4, 1, true, false, 4, true>(a, ws, wsb, s);
-        return mw == 2 ? launch_bf<1, 2, true, false, 4, true>(a, ws, wsb, s) : launch_bf<1, 1, true, false, 4, true>(a, ws, wsb, s);
-    } else {
-        if (nt == 4) return mw == 2 ? launch_bf<4, 2, false, false, 2, true>(a, ws, wsb, s) : launch_bf<4, 1, false, false, 2, true>(a, ws, wsb, s);
-        return mw == 2 ? launch_bf<1, 2, false, false, 2, true>(a, ws, wsb, s) : launch_bf<1, 1, false, false, 2, true>(a, ws, wsb, s);
-    }
+    constexpr int D = (F16X2 || !CI8) ? 2 : 4;
+    if (nt == 4)
+        return mw == 2 ? launch_epi<4, 2, CI8, F16X2, D>(a, epi, ws, wsb, s)
+                       : launch_epi<4, 1, CI8, F16X2, D>(a, epi, ws, wsb, s);
+    return mw == 2 ? launch_epi<1, 2, CI8, F16X2, D>(a, epi, ws, wsb, s)
+                   : launch_epi<1, 1, CI8, F16X2, D>(a, epi, ws, wsb, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -873,7 +874,7 @@ static int plan_batch(bool f16x2, const blade_beamform_block* blocks, int n,
             return fail("batched beamformer: every block needs A, F, T, B > 0");
         if (bk.A > 4096 || bk.F > 65535 || bk.T > (1u << 30) || bk.B > 4096)
             return fail("batched beamformer: block shape too large");
-        if (bk.T % NT) return fail("batched beamformer: T must be a multiple of 4");
+        if (bk.T % (TILE * NT)) return fail("batched beamformer: T must be a multiple of 64");
         if (!bk.input || !bk.phasors || !bk.output) return fail("batched beamformer: null pointer");
         if ((uintptr_t)bk.input % 16 || (uintptr_t)bk.phasors % 16 || (uintptr_t)bk.output % 16)
             return fail("batched beamformer: buffers must be 16-byte aligned");
@@ -995,14 +996,7 @@ static int run_beamformer(int in_type, int precision, const void* input, const v
     const int epi = (int)rate | (polarize ? 8 : 0) | (incoherent ? 16 : 0);
     if (precision == BLADE_PREC_F16X2) return dispatch<true, true>(a, T % 4 == 0 ? 4 : 1, mw, epi, ws, wsb, s);
     if (in_type == BLADE_CI8) {
-        // NT = 4 (139 VGPRs, 3 waves per SIMD) hides the drain of each
-        // chunk's beam stores, which every later load wait includes (vmcnt
-        // counts stores too); NT = 8 (two runs of 4 per lane, a wave spanning
-        // 128 samples that must not straddle T) reuses phasors more but runs
-        // 2 waves per SIMD.  BLADE_BF_NT=8 selects it.
-        int nt = T % 4 == 0 ? 4 : 1;
-        if (env_int("BLADE_BF_NT", 0) == 8 && T % 128 == 0) nt = 8;
-        return dispatch<true, false>(a, nt, mw, epi, ws, wsb, s);
+        return dispatch<true, false>(a, T % 4 == 0 ? 4 : 1, mw, epi, ws, wsb, s);
     }
     return dispatch<false, false>(a, T % 4 == 0 ? 4 : 1, mw, epi, ws, wsb, s);
 }

@@ -108,7 +108,7 @@ int blade_beamform_chain_ws(int in_type, int precision, const void* input, const
  * reference counterpart; each block is what one Beamformer::ATA module call
  * computes, src/modules/beamformer/beamformer.cu:13-79, without the
  * incoherent beam).  Each block: CI8 input [A,F,T,2], CF32 phasors
- * [B,A,F,1,2], CF32 output [B,F,T,2]; T a multiple of 4.  The call is
+ * [B,A,F,1,2], CF32 output [B,F,T,2]; T a multiple of 64.  The call is
  * asynchronous on `stream`, allocates nothing, and can be captured in a
  * hipGraph: size the caller-owned workspace with
  * blade_beamform_batched_workspace_size (same blocks, same tuning).
