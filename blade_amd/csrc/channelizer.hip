@@ -364,6 +364,197 @@ __global__ __launch_bounds__(1024) void chan_detect_kernel(const float2* __restr
     }
 }
 
+// ---------------------------------------------------------------------------
+// Power-of-two rates 16 <= C <= 8192 with both polarizations.
+//
+// A spectrum is handled by C/8 threads; a workgroup holds SPW = max(1,
+// 2048/C) spectra.  Pass 1 (radix 8) runs on registers loaded straight from
+// HBM (eight 16-byte loads per thread, both pols, all in flight at once);
+// the radix-8 middle passes run in LDS.  The last pass (radix RL = 2, 4 or 8)
+// reads LDS and writes the fftshifted spectrum (MODE 0) or integrates the
+// detector (MODE 1: Stokes I, MODE 4: XX*, YY*, Re XY*, Im XY*) in registers,
+// so the spectra of the fused channelizer + detector never reach HBM.
+// ---------------------------------------------------------------------------
+template <int MODE, int RL>
+__global__ __launch_bounds__(1024) void fft2_kernel(const float4* __restrict__ in, void* __restrict__ outv,
+                                                    int C, int nmid, uint64_t S, uint64_t n_items,
+                                                    int N, int accumulate,
+                                                    const float2* __restrict__ tw) {
+    extern __shared__ float2 data[];
+    constexpr int G = 8 / RL;                   // last-pass butterflies per thread and pol
+    const int tps = C >> 3;                     // threads per spectrum
+    const int spw = blockDim.x / tps;
+    const int slot = threadIdx.x / tps;
+    const int j = threadIdx.x - slot * tps;
+    const int PC = spw * 2 * C;
+    float2* const blk = data + (size_t)slot * 2 * C;    // pol 0; pol 1 at + C
+    const uint64_t per_round = (uint64_t)gridDim.x * spw;
+    const uint64_t rounds = (n_items + per_round - 1) / per_round;
+    const int nspec = MODE == 0 ? 1 : (int)S;
+    const int groups = MODE == 0 ? 1 : ((N <= (int)S) ? (int)S / N : 1);
+    const int lastR = C / RL;                   // stride of the last pass
+
+    auto item_of = [&](uint64_t round) { return (round * gridDim.x + blockIdx.x) * spw + slot; };
+    auto spec_of = [&](uint64_t round, int s) {
+        const uint64_t it = item_of(round);
+        const uint64_t sp = MODE == 0 ? it : it * S + s;
+        return it < n_items ? sp : 0;           // idle slots re-read spectrum 0
+    };
+    float4 q[8];
+    auto load = [&](uint64_t sp) {
+        const float4* src = in + sp * (uint64_t)C;
+#pragma unroll
+        for (int r = 0; r < 8; r++) q[r] = src[j + r * tps];
+    };
+
+    constexpr int NPOL = MODE == 0 ? 1 : MODE;
+    float acc[G][RL][NPOL];
+#pragma unroll
+    for (int g = 0; g < G; g++)
+#pragma unroll
+        for (int r = 0; r < RL; r++)
+#pragma unroll
+            for (int c = 0; c < NPOL; c++) acc[g][r][c] = 0.f;
+
+    for (uint64_t round = 0; round < rounds; round++) {
+        const uint64_t item = item_of(round);
+        const bool valid = item < n_items;
+        for (int s = 0; s < nspec; s++) {
+            load(spec_of(round, s));
+            // ---- pass 1: radix 8 on registers -> LDS (Stockham, Ns = 1)
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                float2 v[8];
+#pragma unroll
+                for (int r = 0; r < 8; r++)
+                    v[r] = p == 0 ? make_float2(q[r].x, q[r].y) : make_float2(q[r].z, q[r].w);
+                dft_reg<8>(v, tw, 0);
+#pragma unroll
+                for (int r = 0; r < 8; r++) blk[p * C + j * 8 + r] = v[r];
+            }
+            __syncthreads();
+            int Ns = 8;
+            for (int i = 0; i < nmid; i++) {
+                stockham_pass<8>(data, C, PC, Ns, tw);
+                Ns *= 8;
+            }
+            // ---- last pass: radix RL, LDS -> output (Ns = C / RL)
+            const uint64_t sp = MODE == 0 ? item : item * S + s;
+#pragma unroll
+            for (int g = 0; g < G; g++) {
+                const int jb = j + g * tps;
+                float2 x[2][RL];
+#pragma unroll
+                for (int p = 0; p < 2; p++) {
+#pragma unroll
+                    for (int r = 0; r < RL; r++) {
+                        float2 v = blk[p * C + jb + r * lastR];
+                        if (r > 0 && jb > 0) v = cmul(v, tw[jb * r]);
+                        x[p][r] = v;
+                    }
+                    dft_reg<RL>(x[p], tw, C / RL);
+                }
+                if constexpr (MODE == 0) {
+                    if (valid) {
+                        const uint64_t af = sp / S, si = sp - af * S;
+                        float4* out = static_cast<float4*>(outv);
+#pragma unroll
+                        for (int r = 0; r < RL; r++) {
+                            const int k = jb + r * lastR;
+                            const int ko = (k + (C >> 1)) & (C - 1);
+                            out[(af * C + ko) * S + si] = make_float4(x[0][r].x, x[0][r].y, x[1][r].x, x[1][r].y);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < RL; r++) {
+                        const float2 X = x[0][r], Y = x[1][r];
+                        if constexpr (MODE == 1) {
+                            acc[g][r][0] += X.x * X.x + X.y * X.y + Y.x * Y.x + Y.y * Y.y;
+                        } else {
+                            acc[g][r][0] += X.x * X.x + X.y * X.y;
+                            acc[g][r][NPOL > 1 ? 1 : 0] += Y.x * Y.x + Y.y * Y.y;
+                            acc[g][r][NPOL > 2 ? 2 : 0] += X.x * Y.x + X.y * Y.y;
+                            acc[g][r][NPOL > 3 ? 3 : 0] += X.y * Y.x - X.x * Y.y;
+                        }
+                    }
+                }
+            }
+            if constexpr (MODE != 0) {
+                const bool flush = (N <= (int)S) ? ((s + 1) % N == 0) : (s == (int)S - 1);
+                if (flush) {
+                    const int grp = (N <= (int)S) ? s / N : 0;
+                    float* out = static_cast<float*>(outv);
+#pragma unroll
+                    for (int g = 0; g < G; g++)
+#pragma unroll
+                        for (int r = 0; r < RL; r++) {
+                            if (valid) {
+                                const int k = j + g * tps + r * lastR;
+                                const int ko = (k + (C >> 1)) & (C - 1);
+                                float* o = out + ((item * C + ko) * groups + grp) * NPOL;
+#pragma unroll
+                                for (int c = 0; c < NPOL; c++) o[c] = accumulate ? o[c] + acc[g][r][c] : acc[g][r][c];
+                            }
+#pragma unroll
+                            for (int c = 0; c < NPOL; c++) acc[g][r][c] = 0.f;
+                        }
+                }
+            }
+            __syncthreads();   // LDS is rewritten by the next spectrum's pass 1
+        }
+    }
+}
+
+// Passes of fft2_kernel for C = 2^k: radix 8 first, nmid radix-8 middle
+// passes, last radix RL with 3 + 3 nmid + log2(RL) = k.
+struct Fft2Plan {
+    int rl, nmid, threads, spw;
+};
+
+static bool fft2_plan(int C, Fft2Plan* pl) {
+    if (C < 16 || C > 8192 || (C & (C - 1))) return false;
+    int k = 0;
+    while ((1 << k) < C) k++;
+    const int rem = (k - 3) % 3;
+    pl->rl = rem == 0 ? 8 : (rem == 1 ? 2 : 4);
+    const int lr = rem == 0 ? 3 : rem;
+    pl->nmid = (k - 3 - lr) / 3;
+    const int tps = C / 8;
+    pl->threads = tps >= 256 ? tps : 256;
+    pl->spw = pl->threads / tps;
+    return true;
+}
+
+template <int MODE>
+static int launch_fft2(const Fft2Plan& pl, const float2* in, void* out, int C, uint64_t S, uint64_t n_items,
+                       int N, int accumulate, const float2* tw, hipStream_t s) {
+    const size_t lds = sizeof(float2) * 2 * (size_t)C * pl.spw;
+    uint64_t blocks = ceil_div(n_items, pl.spw);
+    const uint64_t cap = 256ull * (pl.threads >= 1024 ? 1 : (pl.threads >= 512 ? 2 : 4)) * 4;
+    if (blocks > cap) blocks = cap;
+    const void* fns[3] = {reinterpret_cast<const void*>(&fft2_kernel<MODE, 2>),
+                          reinterpret_cast<const void*>(&fft2_kernel<MODE, 4>),
+                          reinterpret_cast<const void*>(&fft2_kernel<MODE, 8>)};
+    const void* fn = fns[pl.rl == 2 ? 0 : (pl.rl == 4 ? 1 : 2)];
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) {
+            set_error(std::string("channelizer: LDS attribute: ") + hipGetErrorString(e));
+            return BLADE_HIP_ERROR;
+        }
+    }
+    const float4* in4 = reinterpret_cast<const float4*>(in);
+    const dim3 grid((unsigned)blocks), block((unsigned)pl.threads);
+    if (pl.rl == 2)
+        hipLaunchKernelGGL((fft2_kernel<MODE, 2>), grid, block, lds, s, in4, out, C, pl.nmid, S, n_items, N, accumulate, tw);
+    else if (pl.rl == 4)
+        hipLaunchKernelGGL((fft2_kernel<MODE, 4>), grid, block, lds, s, in4, out, C, pl.nmid, S, n_items, N, accumulate, tw);
+    else
+        hipLaunchKernelGGL((fft2_kernel<MODE, 8>), grid, block, lds, s, in4, out, C, pl.nmid, S, n_items, N, accumulate, tw);
+    return check_launch("channelizer");
+}
+
 // Direct O(C^2) DFT for rates with prime factors above 7.
 template <int P>
 __global__ __launch_bounds__(256) void chan_direct_kernel(const float2* __restrict__ in,
@@ -483,6 +674,9 @@ extern "C" int blade_channelizer(const void* input, void* output, uint64_t A, ui
     const float2* tw = nullptr;
     int rc = get_twiddles(C, s, &tw);
     if (rc) return rc;
+    Fft2Plan pl;
+    if (P == 2 && fft2_plan(C, &pl))
+        return launch_fft2<0>(pl, in, out, C, S, n_spec, 1, 0, tw, s);
     Radices rad;
     const size_t lds = sizeof(float2) * (size_t)C * P;
     uint64_t blocks = n_spec;
@@ -549,6 +743,14 @@ extern "C" int blade_channelizer_detector(const void* input, void* output, uint6
     const float2* tw = nullptr;
     int rc = get_twiddles(C, s, &tw);
     if (rc) return rc;
+    Fft2Plan pl;
+    if (fft2_plan(C, &pl)) {
+        if (out_pols == 1)
+            return launch_fft2<1>(pl, static_cast<const float2*>(input), output, C, S, A * F, (int)integration,
+                                  accumulate, tw, s);
+        return launch_fft2<4>(pl, static_cast<const float2*>(input), output, C, S, A * F, (int)integration,
+                              accumulate, tw, s);
+    }
     int threads = 64;
     while (threads < 1024 && threads * 8 < 2 * C) threads *= 2;
     const size_t lds = sizeof(float2) * 2 * (size_t)C;
