@@ -15,7 +15,9 @@ oracle on a few channels.
 Extra legs (not the headline value; ``--no-extra`` skips them):
 ``batched`` -- 32 uneven blocks in one persistent launch, autotuned;
 ``mode_a`` -- split-FP16 beamform -> FFT-8192 -> Stokes I on A28 F192 T16384 B8;
-``calibration`` -- fused Stokes-I loss/gradient and Adam steps on A42 F384 T2048 B4.
+``calibration`` -- fused Stokes-I loss/gradient and Adam steps on A42 F384 T2048 B4;
+``kernels`` -- achieved HBM rate of the bandwidth-bound kernels (polarizer,
+detector, cast, gather, permutation, channelizer) on the block's beams.
 """
 
 from __future__ import annotations
@@ -257,6 +259,41 @@ def calibration_leg(steps=200):
             "adam_steps": steps, "loss_first_last": [round(losses[0], 4), round(losses[-1], 4)]}
 
 
+def kernels_leg(iters=20):
+    """Achieved HBM rate of the bandwidth-bound kernels of the path at the
+    production block's sizes (algorithmic bytes = bytes read + written once)."""
+    import torch
+    from blade_amd import ops
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    beams = torch.randn((16, 192, 8192, 2), dtype=torch.complex64, device="cuda", generator=gen)
+    nb = beams.numel() * 8
+    x8 = _ci8((28, 192, 8192, 2), gen)
+    out = {}
+
+    def rate(name, fn, nbytes):
+        _events_ms(fn, 3)
+        ms = _events_ms(fn, iters)
+        out[name] = {"us": round(ms * 1e3, 1), "gbs": round(nbytes / ms / 1e6, 1),
+                     "hbm_frac": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 3)}
+
+    pol = torch.empty_like(beams)
+    rate("polarizer_cf32", lambda: ops.polarizer(beams, out=pol), 2 * nb)
+    det = torch.empty((16, 192, 8192 // 4, 1), dtype=torch.float32, device="cuda")
+    rate("detector_stokes_i_int4", lambda: ops.detector(beams, 4, 1, out=det), nb + det.numel() * 4)
+    cf = torch.empty((28, 192, 8192, 2), dtype=torch.complex64, device="cuda")
+    rate("cast_ci8_cf32", lambda: ops.cast(x8, "cf32", out=cf, in_type="ci8"), x8.numel() + cf.numel() * 8)
+    g = torch.empty((16, 192, 8192 * 2, 2), dtype=torch.complex64, device="cuda")
+    rate("gather_x2_time", lambda: ops.gather(beams, g, 2, 2, 0), 2 * nb)
+    rate("permutation_aftp_atpf", lambda: ops.permutation(beams, (0, 2, 3, 1)), 2 * nb)
+    ch = torch.empty_like(beams)
+    for C in (4, 64, 1024, 8192):
+        rate(f"channelizer_c{C}", lambda C=C: ops.channelizer(beams, C, out=ch), 2 * nb)
+    cd = torch.empty((16, 192 * 8192, 1, 1), dtype=torch.float32, device="cuda")
+    rate("channelizer_detector_c8192", lambda: ops.channelizer_detector(beams, 8192, 1, 1, out=cd),
+         nb + cd.numel() * 4)
+    return out
+
+
 def _extra(fn):
     try:
         return fn()
@@ -348,6 +385,7 @@ def main():
         line["batched"] = _extra(batched_leg)
         line["mode_a"] = _extra(mode_a_leg)
         line["calibration"] = _extra(calibration_leg)
+        line["kernels"] = _extra(kernels_leg)
     if not args.no_cpu:
         x, w, outs = bufs
         cpu, errs = cpu_leg(x, w, outs)

@@ -370,14 +370,36 @@ extern "C" int blade_permutation(const void* input, void* output, uint64_t elem_
     uint64_t in_stride[4];
     in_stride[3] = 1;
     for (int d = 2; d >= 0; d--) in_stride[d] = in_stride[d + 1] * in_shape4[d + 1];
-    Perm4 p;
+    uint64_t shape[4], stride[4];
     uint64_t total = 1;
     for (int d = 0; d < 4; d++) {
-        p.out_shape[d] = in_shape4[indexes4[d]];
-        p.in_stride_of_out_dim[d] = in_stride[indexes4[d]];
-        total *= p.out_shape[d];
+        shape[d] = in_shape4[indexes4[d]];
+        stride[d] = in_stride[indexes4[d]];
+        total *= shape[d];
     }
     if (total == 0) return BLADE_SUCCESS;
+    // Canonical form: output axes that are also contiguous and in order in
+    // the input merge into one (AFTP -> ATPF is a transpose of [F][T*P]), and
+    // unit axes drop; the result is padded back to four axes at the front.
+    uint64_t ms[4], mst[4];
+    int n = 0;
+    for (int d = 0; d < 4; d++) {
+        if (shape[d] == 1) continue;
+        if (n > 0 && mst[n - 1] == stride[d] * shape[d]) {
+            ms[n - 1] *= shape[d];
+            mst[n - 1] = stride[d];
+        } else {
+            ms[n] = shape[d];
+            mst[n] = stride[d];
+            n++;
+        }
+    }
+    Perm4 p;
+    for (int d = 0; d < 4; d++) {
+        const int k = d - (4 - n);
+        p.out_shape[d] = k >= 0 ? ms[k] : 1;
+        p.in_stride_of_out_dim[d] = k >= 0 ? mst[k] : 0;
+    }
     if (!input || !output) return fail("permutation: null pointer");
     if (input == output) return fail("permutation: in-place permutation is not supported");
     switch (elem_bytes) {
