@@ -38,7 +38,8 @@ __device__ __forceinline__ void store_out(float* out, uint64_t o, const Pow4& a,
     }
 }
 
-// One lane per output sample.
+// One lane per output sample (integrations of 1 and of odd or other short
+// lengths).
 template <int NPOL>
 __global__ __launch_bounds__(256) void detector_lane_kernel(const float4* __restrict__ in,
                                                             float* __restrict__ out,
@@ -95,6 +96,16 @@ static void launch(const float4* in, float* out, uint64_t n_out, uint64_t intg, 
     } else if (intg >= 16) {
         hipLaunchKernelGGL((detector_group_kernel<NPOL, 16>), dim3(grid_cap(ceil_div(n_out, 16))),
                            dim3(256), 0, s, in, out, n_out, intg, acc);
+    } else if (intg == 8 || intg == 4 || intg == 2) {
+        // One lane per input sample: every load instruction reads 64
+        // consecutive samples; the group of `intg` lanes reduces by shuffles.
+        const unsigned g = grid_cap(ceil_div(n_out * intg, 256));
+        if (intg == 8)
+            hipLaunchKernelGGL((detector_group_kernel<NPOL, 8>), dim3(g), dim3(256), 0, s, in, out, n_out, intg, acc);
+        else if (intg == 4)
+            hipLaunchKernelGGL((detector_group_kernel<NPOL, 4>), dim3(g), dim3(256), 0, s, in, out, n_out, intg, acc);
+        else
+            hipLaunchKernelGGL((detector_group_kernel<NPOL, 2>), dim3(g), dim3(256), 0, s, in, out, n_out, intg, acc);
     } else {
         hipLaunchKernelGGL((detector_lane_kernel<NPOL>), dim3(grid_cap(ceil_div(n_out, 256))),
                            dim3(256), 0, s, in, out, n_out, intg, acc);

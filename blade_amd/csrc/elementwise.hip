@@ -54,7 +54,9 @@ static unsigned grid_for(uint64_t work, unsigned block) {
 
 template <typename IT, typename OT>
 static int launch_cast(const void* in, void* out, uint64_t n_scalars, hipStream_t s) {
-    constexpr int V = 8;
+    // Widening casts: 16 output bytes per lane, so that every store
+    // instruction writes 64 lanes x 16 contiguous bytes.  Others: 8 scalars.
+    constexpr int V = sizeof(OT) > sizeof(IT) ? (16 / sizeof(OT) > 0 ? 16 / sizeof(OT) : 1) : 8;
     const IT* src = static_cast<const IT*>(in);
     OT* dst = static_cast<OT*>(out);
     const bool aligned = ((uintptr_t)src % (sizeof(IT) * V) == 0) &&
