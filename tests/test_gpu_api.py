@@ -427,3 +427,52 @@ def test_phasor_delays_geometry():
     bl.copy(ph, mod.get_phasors())
     ref = O.phasor_ata(d, np.ones((A, F, 1, 2)), 6500.125e6, 0.5e6, 1.024e9, 352)
     assert np.max(np.abs(ph.as_numpy() - ref)) < 2e-7
+
+
+# ------------------------------------------------ Mode-H bundle (mode_h.py)
+
+@bl.runner
+class ModeHPipeline:
+    def __init__(self, in_shape, out_shape, config, dtype):
+        self.input.buffer = bl.array_tensor(in_shape, dtype=dtype)
+        self.output.buffer = bl.array_tensor(out_shape, dtype=bl.f32)
+        self.module.mode_h = bl.module(bl.modeh, config, self.input.buffer, it=dtype, ot=bl.f32)
+
+    def transfer_in(self, buffer):
+        self.copy(self.input.buffer, buffer)
+
+    def transfer_out(self, buffer):
+        self.copy(self.output.buffer, self.module.mode_h.get_output())
+        self.copy(buffer, self.output.buffer)
+
+
+@pytest.mark.parametrize("it,in_shape,circ,intg,npol", [
+    ("cf32", (2, 192, 8192, 2), True, 1, 1),      # tests/bundles/generic/mode_h.py:21-33
+    ("ci8", (2, 192, 8192, 2), True, 1, 1),
+    ("cf32", (3, 16, 2048, 2), False, 1, 4),      # linear feeds, full-Stokes output
+])
+def test_mode_h_bundle(it, in_shape, circ, intg, npol):
+    # Cast -> FFT channelizer (rate = T) -> polarizer -> detector; the
+    # reference test's own input (uniform [0, 1) real and imaginary parts)
+    # for CF32, CI8 Gaussian voltages for the CI8 variant.
+    B, F, T, P = in_shape
+    out_shape = (B, F * T, 1, npol)
+    config = {"input_shape": in_shape, "output_shape": out_shape,
+              "polarizer_convert_to_circular": circ,
+              "detector_integration_size": intg,
+              "detector_number_of_output_polarizations": npol}
+    rng = np.random.default_rng(21)
+    dtype = bl.ci8 if it == "ci8" else bl.cf32
+    if it == "ci8":
+        x = O.ci8_gaussian(in_shape, rng=rng)
+    else:
+        x = crand(rng, in_shape)
+    host_in = bl.array_tensor(in_shape, dtype=dtype, device=bl.cpu)
+    host_out = bl.array_tensor(out_shape, dtype=bl.f32, device=bl.cpu)
+    fill(host_in, x)
+    p = ModeHPipeline(in_shape, out_shape, config, dtype)
+    p(host_in, host_out)
+    ref = O.mode_h(x, convert_to_circular=circ, detector_integration_size=intg,
+                   detector_number_of_output_polarizations=npol)
+    assert ref.shape == out_shape
+    assert O.norm_rel_err(host_out.as_numpy(), ref) <= TOL
