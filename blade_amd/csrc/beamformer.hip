@@ -831,18 +831,6 @@ struct BatchPlan {
     int grid;
 };
 
-static int device_cus() {
-    static int cached[64] = {0};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-    if (!cached[dev]) {
-        int n = 0;
-        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
-        cached[dev] = n;
-    }
-    return cached[dev];
-}
-
 // Plans up to kBatchMax blocks; the workspace pointer may be null (sizing).
 static int plan_batch(bool f16x2, const blade_beamform_block* blocks, int n,
                       const blade_batch_tuning* tuning, unsigned char* ws, BatchPlan* plan) {

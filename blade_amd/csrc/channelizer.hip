@@ -27,6 +27,10 @@ __global__ void twiddle_kernel(float2* tw, int C) {
     }
 }
 
+// Each table is followed by a small sink that idle lanes of the radix-16
+// kernel store into, so that every lane issues the same stores.
+constexpr size_t kTwiddleSinkBytes = 512 * sizeof(float4);
+
 static std::mutex g_tw_mutex;
 static std::map<std::pair<int, int>, float2*> g_tw;
 
@@ -40,7 +44,7 @@ static int get_twiddles(int C, hipStream_t s, const float2** out) {
         return BLADE_SUCCESS;
     }
     float2* p = nullptr;
-    hipError_t e = hipMalloc(&p, sizeof(float2) * C);
+    hipError_t e = hipMalloc(&p, sizeof(float2) * C + kTwiddleSinkBytes);
     if (e != hipSuccess) {
         set_error(std::string("channelizer: twiddle allocation: ") + hipGetErrorString(e));
         return BLADE_HIP_ERROR;
@@ -365,7 +369,8 @@ __global__ __launch_bounds__(1024) void chan_detect_kernel(const float2* __restr
 }
 
 // ---------------------------------------------------------------------------
-// Power-of-two rates 16 <= C <= 8192 with both polarizations.
+// Power-of-two rates 16 < C < 256 with both polarizations (fft16_kernel
+// below takes 256 <= C <= 8192).
 //
 // A spectrum is handled by C/8 threads; a workgroup holds SPW = max(1,
 // 2048/C) spectra.  Pass 1 (radix 8) runs on registers loaded straight from
@@ -555,6 +560,393 @@ static int launch_fft2(const Fft2Plan& pl, const float2* in, void* out, int C, u
     return check_launch("channelizer");
 }
 
+// ---------------------------------------------------------------------------
+// Power-of-two rates 256 <= C <= 8192 with both polarizations, radix 16.
+//
+// Every thread holds 16 points of each polarization, so a spectrum takes
+// C/16 threads and a workgroup of NT threads holds NT*16/C spectra.  The
+// passes are radix 16 (floor(log2 C / 4) of them) plus one trailing radix 2,
+// 4 or 8 pass when log2 C is not a multiple of 4:
+//   first   radix 16 on registers loaded straight from HBM: sixteen 16-byte
+//           loads per thread (both pols), issued one spectrum ahead so that
+//           they are in flight during the LDS passes of the current one;
+//   middle  radix 16, LDS -> LDS;
+//   last    LDS -> the fftshifted spectrum in HBM (MODE 0), or detector sums
+//           in registers (MODE 1: Stokes I; MODE 4: XX*, YY*, Re XY*, Im XY*).
+// LDS holds one plane per polarization with one float2 of padding after every
+// 16 points: the 16 consecutive points a lane writes in the first pass, and
+// the groups of 16 written in the Ns = 16 pass, then fall on distinct banks.
+// Twiddles depend only on the thread's position in its spectrum: W, W^2, W^4
+// and W^8 of each pass are read from the table once per launch and the other
+// powers are products of these, so the loop's only global loads are the next
+// spectrum's samples.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int lpad(int pos) { return pos + (pos >> 4); }
+
+// a * W_16^E with W_16 = exp(-j 2 pi / 16); cos(2 pi e / 16) by table,
+// -sin(2 pi e / 16) = cos(2 pi (e + 4) / 16).
+template <int E>
+__device__ __forceinline__ float2 mul_w16(float2 a) {
+    constexpr int e = E & 15;
+    constexpr float c1 = 0.92387953251128675613f;   // cos(pi/8)
+    constexpr float s1 = 0.38268343236508977173f;   // sin(pi/8)
+    constexpr float h = 0.70710678118654752440f;    // cos(pi/4)
+    constexpr float cs[16] = {1.f, c1, h, s1, 0.f, -s1, -h, -c1, -1.f, -c1, -h, -s1, 0.f, s1, h, c1};
+    if constexpr (e == 0) return a;
+    else if constexpr (e == 4) return make_float2(a.y, -a.x);
+    else if constexpr (e == 8) return make_float2(-a.x, -a.y);
+    else if constexpr (e == 12) return make_float2(-a.y, a.x);
+    else return cmul(a, make_float2(cs[e], cs[(e + 4) & 15]));
+}
+
+// 16-point DFT in natural order: n = 4 n1 + n2, k = k1 + 4 k2,
+// X[k1 + 4 k2] = sum_n2 W4^(n2 k2) W16^(n2 k1) sum_n1 W4^(n1 k1) x[4 n1 + n2].
+__device__ __forceinline__ void dft16(float2 (&v)[16]) {
+    float2 y[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; n2++) {
+        float2 t[4] = {v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]};
+        dft_reg<4>(t, nullptr, 0);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; k1++) y[n2][k1] = t[k1];
+    }
+    y[1][1] = mul_w16<1>(y[1][1]); y[1][2] = mul_w16<2>(y[1][2]); y[1][3] = mul_w16<3>(y[1][3]);
+    y[2][1] = mul_w16<2>(y[2][1]); y[2][2] = mul_w16<4>(y[2][2]); y[2][3] = mul_w16<6>(y[2][3]);
+    y[3][1] = mul_w16<3>(y[3][1]); y[3][2] = mul_w16<6>(y[3][2]); y[3][3] = mul_w16<9>(y[3][3]);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) {
+        float2 t[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+        dft_reg<4>(t, nullptr, 0);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; k2++) v[k1 + 4 * k2] = t[k2];
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void dft_pow2(float2 (&v)[R]) {
+    if constexpr (R == 16) dft16(v);
+    else dft_reg<R>(v, nullptr, 0);
+}
+
+// w[r] = W^r for r < R from b = {W, W^2, W^4, W^8}.
+template <int R>
+__device__ __forceinline__ void twiddle_powers(const float2 (&b)[4], float2 (&w)[R]) {
+    w[0] = make_float2(1.f, 0.f);
+    if constexpr (R > 1) w[1] = b[0];
+    if constexpr (R > 2) { w[2] = b[1]; w[3] = cmul(b[0], b[1]); }
+    if constexpr (R > 4) {
+        w[4] = b[2];
+#pragma unroll
+        for (int r = 5; r < 8; r++) w[r] = cmul(w[r - 4], b[2]);
+    }
+    if constexpr (R > 8) {
+        w[8] = b[3];
+#pragma unroll
+        for (int r = 9; r < 16; r++) w[r] = cmul(w[r - 8], b[3]);
+    }
+}
+
+// One radix-16 Stockham pass in LDS, Ns = NS, butterfly j = tl.
+template <int C, int NS>
+__device__ __forceinline__ void fft16_mid(float2* __restrict__ p0, float2* __restrict__ p1, int tl,
+                                          const float2 (&tb)[4]) {
+    constexpr int STRIDE = C / 16;
+    float2 a[16], b[16];
+    const int rd = lpad(tl);
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        a[r] = p0[rd + r * (STRIDE + STRIDE / 16)];
+        b[r] = p1[rd + r * (STRIDE + STRIDE / 16)];
+    }
+    float2 w[16];
+    twiddle_powers<16>(tb, w);
+#pragma unroll
+    for (int r = 1; r < 16; r++) {
+        a[r] = cmul(a[r], w[r]);
+        b[r] = cmul(b[r], w[r]);
+    }
+    dft16(a);
+    dft16(b);
+    __syncthreads();
+    const int jm = tl & (NS - 1);
+    const int wr = lpad((tl - jm) * 16 + jm);
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        p0[wr + r * (NS + NS / 16)] = a[r];
+        p1[wr + r * (NS + NS / 16)] = b[r];
+    }
+    __syncthreads();
+}
+
+template <int C, int NMID, int I>
+__device__ __forceinline__ void fft16_mids(float2* p0, float2* p1, int tl, const float2 (&tm)[NMID > 0 ? NMID : 1][4]) {
+    if constexpr (I < NMID) {
+        constexpr int NS = 1 << (4 * (I + 1));
+        fft16_mid<C, NS>(p0, p1, tl, tm[I]);
+        fft16_mids<C, NMID, I + 1>(p0, p1, tl, tm);
+    }
+}
+
+template <int LC, int NT, int MODE>
+__global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__ in, void* __restrict__ outv,
+                                                      uint64_t S, uint64_t n_items, int N, int accumulate,
+                                                      const float2* __restrict__ tw, float4* __restrict__ sink) {
+    constexpr int C = 1 << LC;
+    constexpr int TPS = C / 16;                        // threads per spectrum
+    static_assert(NT % TPS == 0 && LC >= 8, "workgroup must hold whole spectra");
+    constexpr int SPW = NT / TPS;                      // spectra per workgroup
+    constexpr int RL = 1 << (LC % 4);                  // trailing radix, 1 = none
+    constexpr int NMID = LC / 4 - (RL > 1 ? 1 : 2);    // radix-16 passes in LDS
+    constexpr int R = RL > 1 ? RL : 16;                // radix of the last pass
+    constexpr int G = 16 / R;                          // last-pass butterflies per thread
+    constexpr int NSL = C / R;                         // Ns (and stride) of the last pass
+    constexpr int PLANE = C + C / 16;
+    constexpr int NPOL = MODE == 0 ? 1 : MODE;
+    extern __shared__ float2 lds[];
+    const int slot = threadIdx.x / TPS;
+    const int tl = threadIdx.x % TPS;
+    float2* const p0 = lds + slot * 2 * PLANE;
+    float2* const p1 = p0 + PLANE;
+
+    // Per-thread twiddle bases.  Middle pass i has Ns = 16^(i+1) and uses
+    // W_{16 Ns}^(tl mod Ns) = tw[(tl mod Ns) * C / (16 Ns)]; the last pass uses
+    // W_C^(tl r) W_16^(g r) for butterfly j = tl + g * C/16.
+    float2 tm[NMID > 0 ? NMID : 1][4];
+#pragma unroll
+    for (int i = 0; i < NMID; i++) {
+        const int ns = 1 << (4 * (i + 1));
+        const int m = (tl & (ns - 1)) * (C / (16 * ns));
+#pragma unroll
+        for (int e = 0; e < 4; e++) tm[i][e] = tw[(m << e) & (C - 1)];
+    }
+    float2 tb[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) tb[e] = tw[(tl << e) & (C - 1)];
+
+    const uint64_t per_round = (uint64_t)gridDim.x * SPW;
+    const uint64_t rounds = (n_items + per_round - 1) / per_round;
+    const int nspec = MODE == 0 ? 1 : (int)S;
+    const int groups = MODE == 0 ? 1 : ((N <= (int)S) ? (int)S / N : 1);
+
+    // The spectra of a workgroup in one round are consecutive (items first ..
+    // first + SPW - 1): addresses are a workgroup-uniform base plus a 32-bit
+    // lane offset.  Idle slots re-read their workgroup's first spectrum (or
+    // spectrum 0) and store nothing.
+    const unsigned SC = (unsigned)(MODE == 0 ? 1 : S) * C;      // float4s per item
+    float4 q[16];
+    auto load = [&](uint64_t round, int s) {
+        const uint64_t first = (round * gridDim.x + blockIdx.x) * SPW;
+        const bool any = first < n_items;
+        const uint64_t b0 = any ? (MODE == 0 ? first * C : (first * S + s) * C) : 0;
+        const unsigned lo = ((any && first + slot < n_items) ? slot * SC : 0u) + tl;
+#pragma unroll
+        for (int r = 0; r < 16; r++) q[r] = (in + (b0 + r * TPS))[lo];
+    };
+
+    float acc[16][NPOL];
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+#pragma unroll
+        for (int c = 0; c < NPOL; c++) acc[i][c] = 0.f;
+
+    load(0, 0);
+    // Nothing pending at loop entry, so the first pass of every spectrum
+    // waits only for its own loads, not for the previous spectrum's stores.
+    __builtin_amdgcn_s_waitcnt(0);
+    for (uint64_t round = 0; round < rounds; round++) {
+        const uint64_t first = (round * gridDim.x + blockIdx.x) * SPW;
+        const uint64_t item = first + slot;
+        const bool valid = item < n_items;
+        for (int s = 0; s < nspec; s++) {
+            // ---- first pass: radix 16 on registers -> LDS (Ns = 1)
+            {
+                float2 a[16], b[16];
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    a[r] = make_float2(q[r].x, q[r].y);
+                    b[r] = make_float2(q[r].z, q[r].w);
+                }
+                dft16(a);
+                dft16(b);
+                const int wr = tl * 17;
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    p0[wr + r] = a[r];
+                    p1[wr + r] = b[r];
+                }
+                // Next spectrum into the registers just consumed (a fresh set
+                // would be copied back at the loop latch behind a full wait).
+                // After the last spectrum this re-reads spectrum 0 (an L2 hit)
+                // rather than branching, which keeps the waits exact.
+                uint64_t nr = round;
+                int ns = s + 1;
+                if (ns == nspec) { ns = 0; nr++; }
+                load(nr, ns);
+            }
+            __syncthreads();
+            fft16_mids<C, NMID, 0>(p0, p1, tl, tm);
+            // ---- last pass: radix R, LDS -> output (Ns = C / R)
+            float2 a[16], b[16];
+            const int rd = lpad(tl);
+#pragma unroll
+            for (int g = 0; g < G; g++)
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int off = (g * TPS + r * NSL) + (g * TPS + r * NSL) / 16;
+                    a[g * R + r] = p0[rd + off];
+                    b[g * R + r] = p1[rd + off];
+                }
+            {
+                float2 w[R];
+                twiddle_powers<R>(tb, w);
+#pragma unroll
+                for (int g = 0; g < G; g++) {
+#pragma unroll
+                    for (int r = 1; r < R; r++) {
+                        float2 wr = w[r];
+                        switch ((g * r) & 15) {     // W_16^(g r), resolved at compile time
+                            case 0: break;
+                            case 1: wr = mul_w16<1>(wr); break;
+                            case 2: wr = mul_w16<2>(wr); break;
+                            case 3: wr = mul_w16<3>(wr); break;
+                            case 4: wr = mul_w16<4>(wr); break;
+                            case 5: wr = mul_w16<5>(wr); break;
+                            case 6: wr = mul_w16<6>(wr); break;
+                            case 7: wr = mul_w16<7>(wr); break;
+                            case 8: wr = mul_w16<8>(wr); break;
+                            case 9: wr = mul_w16<9>(wr); break;
+                            case 10: wr = mul_w16<10>(wr); break;
+                            case 11: wr = mul_w16<11>(wr); break;
+                            case 12: wr = mul_w16<12>(wr); break;
+                            case 13: wr = mul_w16<13>(wr); break;
+                            case 14: wr = mul_w16<14>(wr); break;
+                            default: wr = mul_w16<15>(wr); break;
+                        }
+                        a[g * R + r] = cmul(a[g * R + r], wr);
+                        b[g * R + r] = cmul(b[g * R + r], wr);
+                    }
+                    float2 xa[R], xb[R];
+#pragma unroll
+                    for (int r = 0; r < R; r++) { xa[r] = a[g * R + r]; xb[r] = b[g * R + r]; }
+                    dft_pow2<R>(xa);
+                    dft_pow2<R>(xb);
+#pragma unroll
+                    for (int r = 0; r < R; r++) { a[g * R + r] = xa[r]; b[g * R + r] = xb[r]; }
+                }
+            }
+            // Channel k = tl + K with K = g C/16 + r C/R; fftshifted
+            // ko = tl + ((K + C/2) mod C), the bracket a multiple of C/16.
+            // Idle lanes store into the sink (same store count on every lane
+            // and path, so waits on the next spectrum's loads stay exact).
+            if constexpr (MODE == 0) {
+                const uint64_t af = item / S, si = item - af * S;
+                float4* const lb = valid ? static_cast<float4*>(outv) + (af * C * S + si) + (uint64_t)tl * S
+                                         : sink + threadIdx.x;
+                const uint64_t step = valid ? S : 0;
+#pragma unroll
+                for (int g = 0; g < G; g++)
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        const int kb = (g * TPS + r * NSL + (C >> 1)) & (C - 1);
+                        const float2 X = a[g * R + r], Y = b[g * R + r];
+                        lb[(uint64_t)kb * step] = make_float4(X.x, X.y, Y.x, Y.y);
+                    }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 16; i++) {
+                    const float2 X = a[i], Y = b[i];
+                    if constexpr (MODE == 1) {
+                        acc[i][0] += X.x * X.x + X.y * X.y + Y.x * Y.x + Y.y * Y.y;
+                    } else {
+                        acc[i][0] += X.x * X.x + X.y * X.y;
+                        acc[i][NPOL > 1 ? 1 : 0] += Y.x * Y.x + Y.y * Y.y;
+                        acc[i][NPOL > 2 ? 2 : 0] += X.x * Y.x + X.y * Y.y;
+                        acc[i][NPOL > 3 ? 3 : 0] += X.y * Y.x - X.x * Y.y;
+                    }
+                }
+                const bool flush = (N <= (int)S) ? ((s + 1) % N == 0) : (s == (int)S - 1);
+                if (flush) {
+                    const int grp = (N <= (int)S) ? s / N : 0;
+                    const unsigned gn = (unsigned)groups * NPOL;
+                    float* const lb = valid ? static_cast<float*>(outv) + (item * C + tl) * gn + grp * NPOL
+                                            : reinterpret_cast<float*>(sink + threadIdx.x);
+                    const unsigned step = valid ? gn : 0;
+#pragma unroll
+                    for (int g = 0; g < G; g++)
+#pragma unroll
+                        for (int r = 0; r < R; r++) {
+                            const int kb = (g * TPS + r * NSL + (C >> 1)) & (C - 1);
+                            float* const o = lb + (uint64_t)((unsigned)kb * step);
+#pragma unroll
+                            for (int c = 0; c < NPOL; c++)
+                                o[c] = accumulate ? o[c] + acc[g * R + r][c] : acc[g * R + r][c];
+                        }
+#pragma unroll
+                    for (int i = 0; i < 16; i++)
+#pragma unroll
+                        for (int c = 0; c < NPOL; c++) acc[i][c] = 0.f;
+                }
+            }
+            __syncthreads();   // LDS is rewritten by the next spectrum's first pass
+        }
+    }
+}
+
+// Launch of fft16_kernel for C = 2^LC; NT threads use NT * 272 bytes of LDS
+// (one 512-thread workgroup per CU).
+template <int LC, int NT, int MODE>
+static int launch_fft16_t(const float2* in, void* out, uint64_t S, uint64_t n_items, int N, int accumulate,
+                          const float2* tw, hipStream_t s) {
+    constexpr int C = 1 << LC;
+    constexpr int SPW = NT / (C / 16);
+    constexpr size_t lds = sizeof(float2) * 2 * (size_t)(C + C / 16) * SPW;
+    const void* fn = reinterpret_cast<const void*>(&fft16_kernel<LC, NT, MODE>);
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) {
+            set_error(std::string("channelizer: LDS attribute: ") + hipGetErrorString(e));
+            return BLADE_HIP_ERROR;
+        }
+    }
+    const uint64_t nb = ceil_div(n_items, SPW);
+    const uint64_t resident = (uint64_t)device_cus() * (NT >= 512 ? 1 : 2);
+    const uint64_t rounds = ceil_div(nb, resident);
+    const uint64_t blocks = ceil_div(nb, rounds);      // equal rounds for every workgroup
+    float4* sink = reinterpret_cast<float4*>(const_cast<float2*>(tw) + C);
+    hipLaunchKernelGGL((fft16_kernel<LC, NT, MODE>), dim3((unsigned)blocks), dim3(NT), lds, s,
+                       reinterpret_cast<const float4*>(in), out, S, n_items, N, accumulate, tw, sink);
+    return check_launch("channelizer");
+}
+
+// fft16_kernel covers the power-of-two rates 256 <= C <= 8192.
+static bool use_fft16(int C) { return C >= 256 && C <= 8192 && !(C & (C - 1)); }
+
+template <int LC, int MODE>
+static int launch_fft16_lc(const float2* in, void* out, uint64_t S, uint64_t n_items, int N, int accumulate,
+                           const float2* tw, hipStream_t s) {
+    return launch_fft16_t<LC, 512, MODE>(in, out, S, n_items, N, accumulate, tw, s);
+}
+
+constexpr uint64_t NT_MAX_SPECTRA = 32;    // spectra per workgroup at C = 256, 512 threads
+
+// Returns 1 and launches when fft16_kernel covers C, else 0.
+template <int MODE>
+static int launch_fft16(int C, const float2* in, void* out, uint64_t S, uint64_t n_items, int N, int accumulate,
+                        const float2* tw, hipStream_t s, int* rc) {
+    // Lane offsets within a workgroup's spectra are 32-bit.
+    if ((uint64_t)(NT_MAX_SPECTRA + 2) * S * (uint64_t)C >= (1ull << 31)) return 0;
+    switch (C) {
+        case 1 << 8: *rc = launch_fft16_lc<8, MODE>(in, out, S, n_items, N, accumulate, tw, s); return 1;
+        case 1 << 9: *rc = launch_fft16_lc<9, MODE>(in, out, S, n_items, N, accumulate, tw, s); return 1;
+        case 1 << 10: *rc = launch_fft16_lc<10, MODE>(in, out, S, n_items, N, accumulate, tw, s); return 1;
+        case 1 << 11: *rc = launch_fft16_lc<11, MODE>(in, out, S, n_items, N, accumulate, tw, s); return 1;
+        case 1 << 12: *rc = launch_fft16_lc<12, MODE>(in, out, S, n_items, N, accumulate, tw, s); return 1;
+        case 1 << 13: *rc = launch_fft16_lc<13, MODE>(in, out, S, n_items, N, accumulate, tw, s); return 1;
+    }
+    return 0;
+}
+
 // Direct O(C^2) DFT for rates with prime factors above 7.
 template <int P>
 __global__ __launch_bounds__(256) void chan_direct_kernel(const float2* __restrict__ in,
@@ -674,6 +1066,10 @@ extern "C" int blade_channelizer(const void* input, void* output, uint64_t A, ui
     const float2* tw = nullptr;
     int rc = get_twiddles(C, s, &tw);
     if (rc) return rc;
+    if (P == 2 && use_fft16(C)) {
+        int rc16 = 0;
+        if (launch_fft16<0>(C, in, out, S, n_spec, 1, 0, tw, s, &rc16)) return rc16;
+    }
     Fft2Plan pl;
     if (P == 2 && fft2_plan(C, &pl))
         return launch_fft2<0>(pl, in, out, C, S, n_spec, 1, 0, tw, s);
@@ -743,6 +1139,13 @@ extern "C" int blade_channelizer_detector(const void* input, void* output, uint6
     const float2* tw = nullptr;
     int rc = get_twiddles(C, s, &tw);
     if (rc) return rc;
+    if (use_fft16(C)) {
+        int rc16 = 0;
+        const float2* in16 = static_cast<const float2*>(input);
+        if (out_pols == 1 ? launch_fft16<1>(C, in16, output, S, A * F, (int)integration, accumulate, tw, s, &rc16)
+                          : launch_fft16<4>(C, in16, output, S, A * F, (int)integration, accumulate, tw, s, &rc16))
+            return rc16;
+    }
     Fft2Plan pl;
     if (fft2_plan(C, &pl)) {
         if (out_pols == 1)

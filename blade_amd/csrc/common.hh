@@ -38,6 +38,19 @@ inline int check_launch(const char* what) {
 
 inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
+// Compute units of the current device (cached per device).
+inline int device_cus() {
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cached[dev]) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cached[dev] = n;
+    }
+    return cached[dev];
+}
+
 // Library-owned scratch for launches that need it, one buffer per (device,
 // stream, consumer) so that calls on different streams never share one;
 // grown on demand (the old buffer is released after the stream drains).

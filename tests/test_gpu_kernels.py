@@ -174,6 +174,39 @@ def test_channelizer(A, F, T, P, C):
     assert O.norm_rel_err(y, ref) <= TOL
 
 
+POW2_RATES = [256, 512, 1024, 2048, 4096, 8192]
+
+
+@pytest.mark.parametrize("C", POW2_RATES)
+def test_channelizer_power_of_two_rates(C):
+    # Every radix-16 plan (16^k times 1, 2, 4 or 8); 42 spectra, two per
+    # (antenna, channel) so the spectra interleave in the output, and a
+    # count that does not fill the last workgroup.
+    rng = np.random.default_rng(C + 1)
+    x = crand(rng, 3, 7, 2 * C, 2)
+    y = host(ops.channelizer(dev(x), C))
+    assert O.norm_rel_err(y, O.channelizer(x, C)) <= TOL
+
+
+@pytest.mark.parametrize("C", POW2_RATES)
+@pytest.mark.parametrize("npol", [1, 4])
+def test_channelizer_detector_power_of_two_rates(C, npol):
+    rng = np.random.default_rng(C + npol)
+    x = crand(rng, 2, 3, 4 * C, 2)
+    y = host(ops.channelizer_detector(dev(x), C, 2, npol))
+    ref = O.detector(O.channelizer(x, C), 2, npol)
+    assert y.shape == ref.shape
+    assert O.norm_rel_err(y, ref) <= TOL
+
+
+def test_channelizer_in_place_rate_equals_t_large():
+    rng = np.random.default_rng(4)
+    x = crand(rng, 2, 5, 8192, 2)
+    t = dev(x)
+    ops.channelizer(t, 8192, out=t)
+    assert O.norm_rel_err(host(t).reshape(2, 5 * 8192, 1, 2), O.channelizer(x, 8192)) <= TOL
+
+
 def test_channelizer_in_place_rate_equals_t():
     rng = np.random.default_rng(3)
     x = crand(rng, 2, 3, 1024, 2)
