@@ -688,7 +688,7 @@ __device__ __forceinline__ void fft16_mids(float2* p0, float2* p1, int tl, const
     }
 }
 
-template <int LC, int NT, int MODE>
+template <int LC, int NT, int MODE, bool NTS>
 __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__ in, void* __restrict__ outv,
                                                       uint64_t S, uint64_t n_items, int N, int accumulate,
                                                       const float2* __restrict__ tw, float4* __restrict__ sink) {
@@ -850,7 +850,19 @@ __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__
                     for (int r = 0; r < R; r++) {
                         const int kb = (g * TPS + r * NSL + (C >> 1)) & (C - 1);
                         const float2 X = a[g * R + r], Y = b[g * R + r];
-                        lb[(uint64_t)kb * step] = make_float4(X.x, X.y, Y.x, Y.y);
+                        float4* const o = lb + (uint64_t)kb * step;
+                        if constexpr (NTS) {
+                            // Component-wise: merged into one nontemporal
+                            // global_store_dwordx4 (a vector operand here is
+                            // lowered through private memory).
+                            float* const of = reinterpret_cast<float*>(o);
+                            __builtin_nontemporal_store(X.x, of);
+                            __builtin_nontemporal_store(X.y, of + 1);
+                            __builtin_nontemporal_store(Y.x, of + 2);
+                            __builtin_nontemporal_store(Y.y, of + 3);
+                        } else {
+                            *o = make_float4(X.x, X.y, Y.x, Y.y);
+                        }
                     }
             } else {
 #pragma unroll
@@ -895,13 +907,13 @@ __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__
 
 // Launch of fft16_kernel for C = 2^LC; NT threads use NT * 272 bytes of LDS
 // (one 512-thread workgroup per CU).
-template <int LC, int NT, int MODE>
+template <int LC, int NT, int MODE, bool NTS>
 static int launch_fft16_t(const float2* in, void* out, uint64_t S, uint64_t n_items, int N, int accumulate,
                           const float2* tw, hipStream_t s) {
     constexpr int C = 1 << LC;
     constexpr int SPW = NT / (C / 16);
     constexpr size_t lds = sizeof(float2) * 2 * (size_t)(C + C / 16) * SPW;
-    const void* fn = reinterpret_cast<const void*>(&fft16_kernel<LC, NT, MODE>);
+    const void* fn = reinterpret_cast<const void*>(&fft16_kernel<LC, NT, MODE, NTS>);
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) {
@@ -914,7 +926,7 @@ static int launch_fft16_t(const float2* in, void* out, uint64_t S, uint64_t n_it
     const uint64_t rounds = ceil_div(nb, resident);
     const uint64_t blocks = ceil_div(nb, rounds);      // equal rounds for every workgroup
     float4* sink = reinterpret_cast<float4*>(const_cast<float2*>(tw) + C);
-    hipLaunchKernelGGL((fft16_kernel<LC, NT, MODE>), dim3((unsigned)blocks), dim3(NT), lds, s,
+    hipLaunchKernelGGL((fft16_kernel<LC, NT, MODE, NTS>), dim3((unsigned)blocks), dim3(NT), lds, s,
                        reinterpret_cast<const float4*>(in), out, S, n_items, N, accumulate, tw, sink);
     return check_launch("channelizer");
 }
@@ -925,7 +937,13 @@ static bool use_fft16(int C) { return C >= 256 && C <= 8192 && !(C & (C - 1)); }
 template <int LC, int MODE>
 static int launch_fft16_lc(const float2* in, void* out, uint64_t S, uint64_t n_items, int N, int accumulate,
                            const float2* tw, hipStream_t s) {
-    return launch_fft16_t<LC, 512, MODE>(in, out, S, n_items, N, accumulate, tw, s);
+    // Nontemporal stores pay off only when each spectrum's output is
+    // contiguous (one spectrum per (antenna, channel)); with interleaved
+    // spectra they turn the partial-line writes into much slower ones.
+    if constexpr (MODE == 0) {
+        if (S == 1) return launch_fft16_t<LC, 512, MODE, true>(in, out, S, n_items, N, accumulate, tw, s);
+    }
+    return launch_fft16_t<LC, 512, MODE, false>(in, out, S, n_items, N, accumulate, tw, s);
 }
 
 constexpr uint64_t NT_MAX_SPECTRA = 32;    // spectra per workgroup at C = 256, 512 threads
