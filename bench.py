@@ -341,9 +341,12 @@ def main():
     achieved_gbs = bytes_ / t_k / 1e9
     t_step = f32["step_ms"] * 1e-3
     value = B * F * T * P / t_step / 1e6
-    # The FP32 kernel sits at the ridge: report the bound whose lower-bound time is larger.
+    # Report the bound whose lower-bound time is larger.  With 16 beams per
+    # wavefront the FP32 kernel forms three real products per complex one,
+    # so its matrix cores execute 6 of the 8 algorithmic flops of a complex
+    # multiply-add: the HBM floor is the higher one.
     t_hbm = bytes_ / (HBM_PEAK_GBS * 1e9)
-    t_mfma = flops / (FP32_PEAK_TFLOPS * 1e12)
+    t_mfma = 0.75 * flops / (FP32_PEAK_TFLOPS * 1e12)
     if t_mfma >= t_hbm:
         roof = {"bound": "mfma", "achieved": round(achieved_tflops, 2), "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4)}
@@ -351,12 +354,13 @@ def main():
         roof = {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 4)}
     roof["traffic"] = _traffic()
-    roof["kernel"] = "beamform_kernel<4,2,true,false,4,true,12> (NT=4, MW=2, CI8, FP32 MFMA, Mode-B epilogue: FFT-4 + polarize) after bf_pack_kernel<2,false>"
+    roof["kernel"] = "beamform_kernel<4,2,true,false,4,true,12> (NT=4, MW=2, CI8, FP32 MFMA with three real products per complex one, Mode-B epilogue: FFT-4 + polarize) after bf_pack_kernel<2,false>"
     roof["kernel_ms"] = round(f32["kernel_ms"], 5)
     roof["hbm_gbs"] = round(achieved_gbs, 1)
     roof["hbm_frac"] = round(achieved_gbs / HBM_PEAK_GBS, 4)
     roof["alg_bytes"] = bytes_
     roof["alg_flops"] = flops
+    roof["alg_tflops"] = round(achieved_tflops, 2)
 
     h = res["f16x2"]
     t_h = h["kernel_ms"] * 1e-3

@@ -7,8 +7,9 @@ shapes, so later runs of the same workload start from the tuned shape.
 
 Candidates span the knobs of the launch:
 
-* ``beam_tiles``: 16-row beam tiles per wavefront (0 = 1 for <= 8 beams,
-  else 2; 2 = always two);
+* ``beam_tiles``: beams per wavefront (0 = 8 for <= 8 beams, else 16;
+  1 = always 8, as (beam, re/im) row pairs; 2 = always 16, where FP32 uses
+  the three-product form);
 * ``chunks_per_item``: 256-sample time chunks per work item (load balance
   against per-item phasor staging);
 * ``workgroups_per_cu``: resident workgroups per compute unit (grid size of
@@ -34,7 +35,7 @@ __all__ = ["candidates", "signature", "tune_batched", "cached_tuning"]
 
 
 def candidates():
-    for bt, cpi, wpc in itertools.product((0, 2), (0, 1, 2, 4, 8), (2, 3, 4)):
+    for bt, cpi, wpc in itertools.product((0, 1, 2), (0, 1, 2, 4, 8), (2, 3, 4)):
         yield {"time_samples_per_lane": 4, "beam_tiles": bt, "chunks_per_item": cpi,
                "workgroups_per_cu": wpc}
 

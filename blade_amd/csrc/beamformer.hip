@@ -22,7 +22,14 @@
 // is multiplied by up to 16 beams.
 //
 // Two arithmetic variants:
-//  * FP32: v_mfma_f32_16x16x4_f32, FP32 accumulate.
+//  * FP32: v_mfma_f32_16x16x4_f32, FP32 accumulate.  With 16 beams per
+//    wavefront (MW = 2) the tile rows are 16 beams and the complex product
+//    takes three real contractions instead of four (Gauss):
+//        P1 = Wr.Xr,  P2 = Wi.Xi,  P3 = (Wr + Wi).(Xr + Xi),
+//        Re y = P1 - P2,  Im y = P3 - P1 - P2,
+//    with Xr + Xi exact for CI8 input and Wr + Wi formed once in the pack
+//    kernel; with 8 beams (MW = 1) the rows are (beam, re/im) pairs of the
+//    embedding below.
 //  * split FP16 (CI8 input only): int8 -> fp16 is exact; each phasor
 //    component is scaled by a per-(workgroup, pol) power of two, split into
 //    hi = fp16(w) and lo = fp16(w - hi), and both halves are multiplied on
@@ -159,6 +166,22 @@ __device__ __forceinline__ void bf_pack_body(const BfArgs& a, const int mp, cons
                 }
             }
             wh[e] = h;
+        }
+    } else if constexpr (MW == 2) {
+        // Three-product layout [p][unit][q][g][row] float, q = (Wr, Wi, Wr + Wi),
+        // row = beam of the 16-beam tile; this workgroup writes rows 8m .. 8m + 7.
+        const int nunits = nsteps / 2;
+        float* wl = reinterpret_cast<float*>(slab) + (size_t)p * nunits * 3 * 64;
+        for (int e = threadIdx.x; e < nunits * 3 * 32; e += blockDim.x) {
+            const int rl = e & 7, kg = (e >> 3) & 3, uq = e >> 5;
+            const int un = uq / 3, q = uq - 3 * un;
+            const int an = un * KG + kg;
+            float v = 0.f;
+            if (an < A) {
+                const float2 w = s_w[rl * A + an];
+                v = q == 0 ? w.x : (q == 1 ? w.y : w.x + w.y);
+            }
+            wl[(un * 3 + q) * 64 + kg * TILE + m * (TILE / 2) + rl] = v;
         }
     } else {
         // [m][p][step][g][row] float, step = 2 * antenna group + component
@@ -433,11 +456,14 @@ __device__ __forceinline__ void bf_body(const BfArgs& a, const int f, const int 
     __syncthreads();
     const float unscale[2] = {s_unscale[0], s_unscale[1]};
 
-    floatx4 acc[MW][2][NT];
+    // Three-product FP32 tiles (16 beams): accumulators P1, P2, P3.
+    constexpr bool G3 = !F16X2 && MW == 2;
+    constexpr int NACC = G3 ? 3 : MW;
+    floatx4 acc[NACC][2][NT];
     float inc[2][NT];
     auto zero_acc = [&]() {
 #pragma unroll
-        for (int m = 0; m < MW; m++)
+        for (int m = 0; m < NACC; m++)
 #pragma unroll
             for (int p = 0; p < 2; p++)
 #pragma unroll
@@ -502,7 +528,7 @@ __device__ __forceinline__ void bf_body(const BfArgs& a, const int f, const int 
             }
         } else {
             const float* wl = reinterpret_cast<const float*>(lds_raw);
-            float xv[2][2][NT];   // [p][c][n]
+            float xv[2][3][NT];   // [p][c][n]: Re x, Im x and (three-product tiles) Re x + Im x
 #pragma unroll
             for (int n = 0; n < NT; n++) {
                 if constexpr (CI8) {
@@ -516,7 +542,20 @@ __device__ __forceinline__ void bf_body(const BfArgs& a, const int f, const int 
 #pragma unroll
                         for (int c = 0; c < 2; c++) xv[p][c][n] = __uint_as_float(u.w[4 * n + 2 * p + c]);
                 }
+#pragma unroll
+                for (int p = 0; p < 2; p++) xv[p][2][n] = xv[p][0][n] + xv[p][1][n];
             }
+            if constexpr (G3) {
+                const int nunits = nsteps / 2;
+#pragma unroll
+                for (int p = 0; p < 2; p++)
+#pragma unroll
+                    for (int q = 0; q < 3; q++) {
+                        const float wv = wl[((p * nunits + un) * 3 + q) * 64 + g * TILE + i];
+#pragma unroll
+                        for (int n = 0; n < NT; n++) acc[q][p][n] = mfma_f32(wv, xv[p][q][n], acc[q][p][n]);
+                    }
+            } else {
 #pragma unroll
             for (int p = 0; p < 2; p++) {
 #pragma unroll
@@ -530,6 +569,7 @@ __device__ __forceinline__ void bf_body(const BfArgs& a, const int f, const int 
 #pragma unroll
                         for (int m = 0; m < MW; m++) acc[m][p][n] = mfma_f32(wv[m], xv[p][c][n], acc[m][p][n]);
                 }
+            }
             }
             const int an = un * KG + g;
             if (do_inc) {
@@ -551,6 +591,22 @@ __device__ __forceinline__ void bf_body(const BfArgs& a, const int f, const int 
         // of the chunk loop (they would pin dozens of registers).
         int fo = f;
         asm volatile("" : "+s"(fo));
+        if constexpr (G3) {
+            // Rows are beams: lane group g holds beams 4g .. 4g + 3.
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int b = b_wg + 4 * g + r;
+                float2 v[2][NT];
+#pragma unroll
+                for (int p = 0; p < 2; p++)
+#pragma unroll
+                    for (int n = 0; n < NT; n++) {
+                        const float p1 = acc[0][p][n][r], p2 = acc[1][p][n][r], p3 = acc[2][p][n][r];
+                        v[p][n] = make_float2(p1 - p2, p3 - p1 - p2);
+                    }
+                store_beam<NT, NTS, EPI>(a, min(b, a.B - 1), fo, t0, v, t_ok && b < a.B);
+            }
+        } else
 #pragma unroll
         for (int m = 0; m < MW; m++) {
 #pragma unroll
@@ -634,7 +690,10 @@ static int env_int(const char* name, int dflt) {
 
 // Bytes of one packed phasor slab and of its A-operand head.
 static void slab_sizes(bool f16x2, int mw, int nsteps, size_t* wbytes, size_t* slab) {
-    *wbytes = f16x2 ? (size_t)mw * 2 * nsteps * 2 * 64 * 16 : (size_t)mw * 2 * nsteps * 64 * 4;
+    // FP32 with two beam tiles stores three products per antenna group
+    // (Wr, Wi, Wr + Wi over 16 beams) instead of the 2 x 2 embedding rows.
+    *wbytes = f16x2 ? (size_t)mw * 2 * nsteps * 2 * 64 * 16
+                    : (mw == 2 ? (size_t)3 * nsteps * 64 * 4 : (size_t)mw * 2 * nsteps * 64 * 4);
     *slab = ((*wbytes + 258 * sizeof(float)) + 255) / 256 * 256;
 }
 

@@ -650,8 +650,11 @@ __device__ __forceinline__ void twiddle_powers(const float2 (&b)[4], float2 (&w)
 // One radix-16 Stockham pass in LDS, Ns = NS, butterfly j = tl.
 template <int C, int NS>
 __device__ __forceinline__ void fft16_mid(float2* __restrict__ p0, float2* __restrict__ p1, int tl,
-                                          const float2 (&tb)[4]) {
+                                          const float2* __restrict__ twl) {
     constexpr int STRIDE = C / 16;
+    float2 tb[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) tb[e] = twl[e * NS + (tl & (NS - 1))];
     float2 a[16], b[16];
     const int rd = lpad(tl);
 #pragma unroll
@@ -679,12 +682,17 @@ __device__ __forceinline__ void fft16_mid(float2* __restrict__ p0, float2* __res
     __syncthreads();
 }
 
+// LDS twiddle table of the middle passes: pass i (Ns = 16^(i+1)) holds
+// W_{16 Ns}^(e' jm) for e' = 1, 2, 4, 8 and jm < Ns, laid out [e][jm] so that
+// lanes with consecutive jm read consecutive entries.
+__host__ __device__ constexpr int fft16_twl_offset(int i) { return i == 0 ? 0 : 4 * 16 + (i - 1) * 4 * 256; }
+
 template <int C, int NMID, int I>
-__device__ __forceinline__ void fft16_mids(float2* p0, float2* p1, int tl, const float2 (&tm)[NMID > 0 ? NMID : 1][4]) {
+__device__ __forceinline__ void fft16_mids(float2* p0, float2* p1, int tl, const float2* twl) {
     if constexpr (I < NMID) {
         constexpr int NS = 1 << (4 * (I + 1));
-        fft16_mid<C, NS>(p0, p1, tl, tm[I]);
-        fft16_mids<C, NMID, I + 1>(p0, p1, tl, tm);
+        fft16_mid<C, NS>(p0, p1, tl, twl + fft16_twl_offset(I));
+        fft16_mids<C, NMID, I + 1>(p0, p1, tl, twl);
     }
 }
 
@@ -709,16 +717,19 @@ __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__
     float2* const p0 = lds + slot * 2 * PLANE;
     float2* const p1 = p0 + PLANE;
 
-    // Per-thread twiddle bases.  Middle pass i has Ns = 16^(i+1) and uses
-    // W_{16 Ns}^(tl mod Ns) = tw[(tl mod Ns) * C / (16 Ns)]; the last pass uses
-    // W_C^(tl r) W_16^(g r) for butterfly j = tl + g * C/16.
-    float2 tm[NMID > 0 ? NMID : 1][4];
+    // Twiddle bases.  Middle pass i has Ns = 16^(i+1) and uses
+    // W_{16 Ns}^(tl mod Ns) = tw[(tl mod Ns) * C / (16 Ns)] and its powers 2,
+    // 4, 8, from an LDS table behind the data planes (filled here, first read
+    // after the first pass's barrier); the last pass uses W_C^(tl r) W_16^(g r)
+    // for butterfly j = tl + g * C/16, from per-thread registers.
+    float2* const twl = lds + SPW * 2 * PLANE;
 #pragma unroll
     for (int i = 0; i < NMID; i++) {
         const int ns = 1 << (4 * (i + 1));
-        const int m = (tl & (ns - 1)) * (C / (16 * ns));
-#pragma unroll
-        for (int e = 0; e < 4; e++) tm[i][e] = tw[(m << e) & (C - 1)];
+        for (int idx = threadIdx.x; idx < 4 * ns; idx += NT) {
+            const int e = idx / ns, jm = idx - e * ns;
+            twl[fft16_twl_offset(i) + idx] = tw[((jm * (C / (16 * ns))) << e) & (C - 1)];
+        }
     }
     float2 tb[4];
 #pragma unroll
@@ -750,15 +761,21 @@ __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__
 #pragma unroll
         for (int c = 0; c < NPOL; c++) acc[i][c] = 0.f;
 
-    load(0, 0);
-    // Nothing pending at loop entry, so the first pass of every spectrum
-    // waits only for its own loads, not for the previous spectrum's stores.
-    __builtin_amdgcn_s_waitcnt(0);
+    // The full-Stokes detector keeps 64 sums per thread, which leaves no
+    // room for the next spectrum's registers: it loads at the top instead.
+    constexpr bool PF = MODE != 4;
+    if constexpr (PF) {
+        load(0, 0);
+        // Nothing pending at loop entry, so the first pass of every spectrum
+        // waits only for its own loads, not for the previous spectrum's stores.
+        __builtin_amdgcn_s_waitcnt(0);
+    }
     for (uint64_t round = 0; round < rounds; round++) {
         const uint64_t first = (round * gridDim.x + blockIdx.x) * SPW;
         const uint64_t item = first + slot;
         const bool valid = item < n_items;
         for (int s = 0; s < nspec; s++) {
+            if constexpr (!PF) load(round, s);
             // ---- first pass: radix 16 on registers -> LDS (Ns = 1)
             {
                 float2 a[16], b[16];
@@ -779,13 +796,15 @@ __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__
                 // would be copied back at the loop latch behind a full wait).
                 // After the last spectrum this re-reads spectrum 0 (an L2 hit)
                 // rather than branching, which keeps the waits exact.
-                uint64_t nr = round;
-                int ns = s + 1;
-                if (ns == nspec) { ns = 0; nr++; }
-                load(nr, ns);
+                if constexpr (PF) {
+                    uint64_t nr = round;
+                    int ns = s + 1;
+                    if (ns == nspec) { ns = 0; nr++; }
+                    load(nr, ns);
+                }
             }
             __syncthreads();
-            fft16_mids<C, NMID, 0>(p0, p1, tl, tm);
+            fft16_mids<C, NMID, 0>(p0, p1, tl, twl);
             // ---- last pass: radix R, LDS -> output (Ns = C / R)
             float2 a[16], b[16];
             const int rd = lpad(tl);
@@ -884,15 +903,46 @@ __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__
                     float* const lb = valid ? static_cast<float*>(outv) + (item * C + tl) * gn + grp * NPOL
                                             : reinterpret_cast<float*>(sink + threadIdx.x);
                     const unsigned step = valid ? gn : 0;
+                    // All earlier sums are read before any store (idle lanes'
+                    // addresses coincide, so the compiler cannot interleave).
+                    // The four full-Stokes sums of a channel are one 16-byte
+                    // access (the output rows are 16-byte aligned).
+                    if (accumulate) {
+                        float old[16][NPOL];
+#pragma unroll
+                        for (int g = 0; g < G; g++)
+#pragma unroll
+                            for (int r = 0; r < R; r++) {
+                                const int kb = (g * TPS + r * NSL + (C >> 1)) & (C - 1);
+                                const float* const o = lb + (uint64_t)((unsigned)kb * step);
+                                if constexpr (NPOL == 4) {
+                                    const float4 v = *reinterpret_cast<const float4*>(o);
+                                    old[g * R + r][0] = v.x;
+                                    old[g * R + r][NPOL > 1 ? 1 : 0] = v.y;
+                                    old[g * R + r][NPOL > 2 ? 2 : 0] = v.z;
+                                    old[g * R + r][NPOL > 3 ? 3 : 0] = v.w;
+                                } else {
+                                    old[g * R + r][0] = o[0];
+                                }
+                            }
+#pragma unroll
+                        for (int i = 0; i < 16; i++)
+#pragma unroll
+                            for (int c = 0; c < NPOL; c++) acc[i][c] += old[i][c];
+                    }
 #pragma unroll
                     for (int g = 0; g < G; g++)
 #pragma unroll
                         for (int r = 0; r < R; r++) {
                             const int kb = (g * TPS + r * NSL + (C >> 1)) & (C - 1);
                             float* const o = lb + (uint64_t)((unsigned)kb * step);
-#pragma unroll
-                            for (int c = 0; c < NPOL; c++)
-                                o[c] = accumulate ? o[c] + acc[g * R + r][c] : acc[g * R + r][c];
+                            const int i = g * R + r;
+                            if constexpr (NPOL == 4)
+                                *reinterpret_cast<float4*>(o) = make_float4(acc[i][0], acc[i][NPOL > 1 ? 1 : 0],
+                                                                            acc[i][NPOL > 2 ? 2 : 0],
+                                                                            acc[i][NPOL > 3 ? 3 : 0]);
+                            else
+                                o[0] = acc[i][0];
                         }
 #pragma unroll
                     for (int i = 0; i < 16; i++)
@@ -912,7 +962,8 @@ static int launch_fft16_t(const float2* in, void* out, uint64_t S, uint64_t n_it
                           const float2* tw, hipStream_t s) {
     constexpr int C = 1 << LC;
     constexpr int SPW = NT / (C / 16);
-    constexpr size_t lds = sizeof(float2) * 2 * (size_t)(C + C / 16) * SPW;
+    constexpr int NMID = LC / 4 - ((LC % 4) ? 1 : 2);
+    constexpr size_t lds = sizeof(float2) * (2 * (size_t)(C + C / 16) * SPW + fft16_twl_offset(NMID));
     const void* fn = reinterpret_cast<const void*>(&fft16_kernel<LC, NT, MODE, NTS>);
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

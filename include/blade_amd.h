@@ -123,7 +123,8 @@ typedef struct blade_beamform_block {
 /*
  * Tile shape and launch shape of the batched launch; 0 = automatic.
  *   time_samples_per_lane: 4
- *   beam_tiles: 16-row beam tiles per wavefront, 1 (8 beams) or 2 (16)
+ *   beam_tiles: 16-row beam tiles per wavefront, 1 (8 beams) or 2 (16;
+ *               FP32 then forms three real products per complex one)
  *   chunks_per_item: time chunks of 256 samples per work item
  *   workgroups_per_cu: resident workgroups per compute unit (grid size)
  */
