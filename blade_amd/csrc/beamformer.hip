@@ -104,7 +104,8 @@ struct BfArgs {
 // One workgroup per (pol p, beam tile m) of one (f, beam group): grid
 // (MW * 2, F, groups).  The split-FP16 scale is shared by the whole beam group
 // of a pol, so that path stages all MW * 8 beams of the pol for the maximum.
-template <int MW, bool F16X2>
+// G3 selects the three-product FP32 layout (16-beam tiles, see bf_body).
+template <int MW, bool F16X2, bool G3 = (!F16X2 && MW == 2)>
 __device__ __forceinline__ void bf_pack_body(const BfArgs& a, const int mp, const int f, const int gz,
                                              const int groups, float2* s_w, unsigned int& s_max) {
     constexpr int AK = F16X2 ? 4 * KG : KG;
@@ -167,7 +168,7 @@ __device__ __forceinline__ void bf_pack_body(const BfArgs& a, const int mp, cons
             }
             wh[e] = h;
         }
-    } else if constexpr (MW == 2) {
+    } else if constexpr (G3) {
         // Three-product layout [p][unit][q][g][row] float, q = (Wr, Wi, Wr + Wi),
         // row = beam of the 16-beam tile; this workgroup writes rows 8m .. 8m + 7.
         const int nunits = nsteps / 2;
@@ -373,7 +374,7 @@ __device__ __forceinline__ void load_words(uint32_t* dst, const uint32_t* src) {
 // One work item: channel f, beam group gz (of `groups`), chunks cw0 ..
 // cw0 + chunks_per_wg - 1 of the block described by `a`.  LDS: lds_raw holds
 // the phasor A operands, s_unscale / s_w0sq the slab's tail.
-template <int NT, int MW, bool CI8, bool F16X2, int D, bool NTS, int EPI>
+template <int NT, int MW, bool CI8, bool F16X2, int D, bool NTS, int EPI, bool G3 = (!F16X2 && MW == 2)>
 __device__ __forceinline__ void bf_body(const BfArgs& a, const int f, const int gz, const int cw0,
                                         const int groups, unsigned char* lds_raw,
                                         float* s_unscale, float (*s_w0sq)[128]) {
@@ -457,7 +458,7 @@ __device__ __forceinline__ void bf_body(const BfArgs& a, const int f, const int 
     const float unscale[2] = {s_unscale[0], s_unscale[1]};
 
     // Three-product FP32 tiles (16 beams): accumulators P1, P2, P3.
-    constexpr bool G3 = !F16X2 && MW == 2;
+    static_assert(!G3 || (!F16X2 && MW == 2), "three products need FP32 and 16-beam tiles");
     constexpr int NACC = G3 ? 3 : MW;
     floatx4 acc[NACC][2][NT];
     float inc[2][NT];
@@ -689,11 +690,11 @@ static int env_int(const char* name, int dflt) {
 }
 
 // Bytes of one packed phasor slab and of its A-operand head.
-static void slab_sizes(bool f16x2, int mw, int nsteps, size_t* wbytes, size_t* slab) {
+static void slab_sizes(bool f16x2, int mw, int nsteps, size_t* wbytes, size_t* slab, bool g3) {
     // FP32 with two beam tiles stores three products per antenna group
     // (Wr, Wi, Wr + Wi over 16 beams) instead of the 2 x 2 embedding rows.
     *wbytes = f16x2 ? (size_t)mw * 2 * nsteps * 2 * 64 * 16
-                    : (mw == 2 ? (size_t)3 * nsteps * 64 * 4 : (size_t)mw * 2 * nsteps * 64 * 4);
+                    : (g3 ? (size_t)3 * nsteps * 64 * 4 : (size_t)mw * 2 * nsteps * 64 * 4);
     *slab = ((*wbytes + 258 * sizeof(float)) + 255) / 256 * 256;
 }
 
@@ -709,7 +710,7 @@ template <int NT, int MW, bool CI8, bool F16X2, int D, bool NTS, int EPI = 0>
 static int launch_bf(const BfArgs& args, unsigned char* ws, size_t ws_bytes, hipStream_t s) {
     BfArgs a = args;
     a.nsteps = nsteps_for(F16X2, a.A);
-    slab_sizes(F16X2, MW, a.nsteps, &a.wbytes, &a.slab_bytes);
+    slab_sizes(F16X2, MW, a.nsteps, &a.wbytes, &a.slab_bytes, !F16X2 && MW == 2);
     const size_t lds = a.wbytes;
     const int span = WAVES * TILE * NT;
     const int groups = groups_for(a.B, MW);
@@ -853,9 +854,9 @@ __global__ __launch_bounds__(256) void bf_pack_batched_kernel(BatchArgs ba) {
     const int gz = local / b.F;
     const BfArgs a = batch_block_args(ba, k);
     if (b.mw == 2)
-        bf_pack_body<2, F16X2>(a, mp, f, gz, b.groups, s_wp, s_max);
+        bf_pack_body<2, F16X2, false>(a, mp, f, gz, b.groups, s_wp, s_max);
     else
-        bf_pack_body<1, F16X2>(a, mp, f, gz, b.groups, s_wp, s_max);
+        bf_pack_body<1, F16X2, false>(a, mp, f, gz, b.groups, s_wp, s_max);
 }
 
 template <int NT, bool F16X2, int D>
@@ -874,9 +875,12 @@ __global__ __launch_bounds__(256, 2) void beamform_batched_kernel(BatchArgs ba) 
         const int tr = local / b.F;
         const BfArgs a = batch_block_args(ba, k);
         __syncthreads();   // the previous item's readers of LDS are done
+        // The four-product FP32 form here: one kernel holds both tile
+        // widths, and the three-product accumulators would set its register
+        // count (and occupancy) for every block.
         if (b.mw == 2)
-            bf_body<NT, 2, true, F16X2, D, true, EPI_PLAIN>(a, f, gz, tr * b.cpw, b.groups, lds_raw,
-                                                            s_unscale, s_w0sq);
+            bf_body<NT, 2, true, F16X2, D, true, EPI_PLAIN, false>(a, f, gz, tr * b.cpw, b.groups, lds_raw,
+                                                                   s_unscale, s_w0sq);
         else
             bf_body<NT, 1, true, F16X2, D, true, EPI_PLAIN>(a, f, gz, tr * b.cpw, b.groups, lds_raw,
                                                             s_unscale, s_w0sq);
@@ -932,7 +936,7 @@ static int plan_batch(bool f16x2, const blade_beamform_block* blocks, int n,
         b.y = static_cast<float2*>(bk.output);
         b.A = (int)bk.A; b.F = (int)bk.F; b.T = (int)bk.T; b.B = (int)bk.B;
         b.nsteps = nsteps_for(f16x2, b.A);
-        slab_sizes(f16x2, b.mw, b.nsteps, &b.wbytes, &b.slab_bytes);
+        slab_sizes(f16x2, b.mw, b.nsteps, &b.wbytes, &b.slab_bytes, false);
         b.groups = groups_for(b.B, b.mw);
         b.n_tchunks = (int)ceil_div(bk.T, span);
         b.cpw = std::min(b.n_tchunks, cpw_req > 0 ? cpw_req : cpw_auto);
@@ -1075,7 +1079,7 @@ extern "C" uint64_t blade_beamformer_workspace_size(int precision, uint64_t A, u
     const bool f16 = precision == BLADE_PREC_F16X2;
     const int mw = B > 8 ? 2 : 1;
     size_t wbytes = 0, slab = 0;
-    slab_sizes(f16, mw, nsteps_for(f16, (int)A), &wbytes, &slab);
+    slab_sizes(f16, mw, nsteps_for(f16, (int)A), &wbytes, &slab, !f16 && mw == 2);
     return (uint64_t)slab * F * groups_for((int)B, mw) + kSinkBytes;
 }
 

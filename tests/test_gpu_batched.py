@@ -1,10 +1,13 @@
 """Batched uneven blocks in one persistent launch, hipGraph capture, autotuner.
 
 The batched launch runs the same per-tile code as the single-block launch,
-so each block's beams must equal a single ``blade_beamformer_ata`` call bit
-for bit; a few channels of every block are also checked against the
-complex128 oracle (normwise relative error <= 1e-5).  Graph replays are
-checked one by one against eager results, with new inputs copied in between.
+so each block's beams equal a single ``blade_beamformer_ata`` call bit for
+bit, except FP32 blocks of more than 8 beams: the single launch forms three
+real products per complex one there and the batched launch four, so those
+agree to rounding (normwise 1e-6).  A few channels of every block are also
+checked against the complex128 oracle (normwise relative error <= 1e-5).
+Graph replays are checked one by one against eager results, with new inputs
+copied in between.
 """
 
 import numpy as np
@@ -28,6 +31,13 @@ def block_shapes(n, seed, Fs=(64, 128, 256, 1024), Ts=(1024, 2048, 4096, 8192)):
     rng = np.random.default_rng(seed)
     return [(int(rng.choice([14, 20, 27, 28])), int(rng.choice(Fs)), int(rng.choice(Ts)),
              int(rng.integers(4, 65))) for _ in range(n)]
+
+
+def same_as_single(got, single, precision, B):
+    """Bit-exact where the batched and single launches share the product form."""
+    if precision == "f16x2" or B <= 8:
+        return np.array_equal(got, single)
+    return O.norm_rel_err(got, single) <= 1e-6
 
 
 def make_blocks(shapes, seed):
@@ -56,7 +66,7 @@ def test_batched_matches_single_launches_and_oracle(precision):
     for k, (x, w, y) in enumerate(blocks):
         single = ops.beamformer(x, w, False, False, precision)
         got = host(y)
-        assert np.array_equal(got, host(single)), f"block {k} {shapes[k]}"
+        assert same_as_single(got, host(single), precision, shapes[k][3]), f"block {k} {shapes[k]}"
         fs = [0, x.shape[1] - 1]
         ref = O.beamformer_ata(host(x[:, fs]), host(w[:, :, fs]))
         assert O.norm_rel_err(got[:, fs], ref) < TOL, f"block {k} {shapes[k]}"
@@ -155,7 +165,7 @@ def test_runner_pipeline_graph_capture():
                                             torch.from_numpy(wi.as_numpy().copy()).cuda(), False, False, "f32")))
         assert p(hin, hph, hout)
         for k in range(len(shapes)):
-            assert np.array_equal(hout[k].as_numpy(), refs[k]), (rep, k)
+            assert same_as_single(hout[k].as_numpy(), refs[k], "f32", shapes[k][3]), (rep, k)
 
 
 def test_autotuner_picks_a_candidate(tmp_path, monkeypatch):
