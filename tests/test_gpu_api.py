@@ -309,12 +309,28 @@ def test_permutation_module(idx):
 
 # ------------------------------------------------------------ bundles (mode_b)
 
+# ITRF positions (m) of ATA antennas 1c 1e 1g 1h 1k 2a 2b 2c 2e 2h 2j 2l 2k
+# 2m 3d 3l 4e 4g 4j 5b, as listed by tests/bundles/ata/mode_bh.py:72-91.
 ATA_POS = [(-2524041.5388905862, -4123587.965024342, 4147646.4222955606),
            (-2524068.187873109, -4123558.735413135, 4147656.21282186),
            (-2524087.2078100787, -4123532.397416349, 4147670.9866770394),
            (-2524103.384010733, -4123511.111598937, 4147682.4133068994),
            (-2524056.730228759, -4123515.287949227, 4147706.4850287656),
-           (-2523986.279601761, -4123497.427940991, 4147766.732988923)]
+           (-2523986.279601761, -4123497.427940991, 4147766.732988923),
+           (-2523970.301363642, -4123515.238502669, 4147758.790023165),
+           (-2523983.5419911123, -4123528.1422073604, 4147737.872218138),
+           (-2523941.5221860334, -4123568.125040547, 4147723.8292249846),
+           (-2524074.096220788, -4123468.5182652213, 4147742.0422435375),
+           (-2524058.6409591637, -4123466.5112451194, 4147753.4513993543),
+           (-2524026.989692545, -4123480.9405167866, 4147758.2356800516),
+           (-2524048.5254066754, -4123468.3463909747, 4147757.835369889),
+           (-2524000.5641107005, -4123498.2984570004, 4147756.815976133),
+           (-2523945.086670364, -4123480.3638816103, 4147808.127865142),
+           (-2523950.6822576034, -4123444.7023326857, 4147839.7474427638),
+           (-2523880.869769226, -4123514.3375464156, 4147813.413426994),
+           (-2523930.3747946257, -4123454.3080821196, 4147842.6449955846),
+           (-2523898.1150373477, -4123456.314794732, 4147860.3045849088),
+           (-2523824.598229116, -4123527.93080514, 4147833.98936114)]
 
 
 def modeb_config(in_shape, out_shape, cal, rate, circ, det):
@@ -475,4 +491,91 @@ def test_mode_h_bundle(it, in_shape, circ, intg, npol):
     ref = O.mode_h(x, convert_to_circular=circ, detector_integration_size=intg,
                    detector_number_of_output_polarizations=npol)
     assert ref.shape == out_shape
+    assert O.norm_rel_err(host_out.as_numpy(), ref) <= TOL
+
+
+# ------------------------------------------ Mode B -> Gather -> Mode H (mode_bh.py)
+
+def test_mode_bh_pipeline_enqueue_dequeue():
+    # tests/bundles/ata/mode_bh.py: Mode-B beams (2 beams, 20 antennas, 512
+    # samples) gathered 4x along time into one Mode-H block (rate = 2048,
+    # circular, Stokes I), driven through enqueue/dequeue until Mode H
+    # outputs.  Mode B runs four steps per Mode-H step (Gather multiplier).
+    A, F, T = 20, 192, 512
+    b_in_shape = (A, F, T, 2)
+    b_out_shape = (2, F, T, 2)
+    h_in_shape = (2, F, 4 * T, 2)
+    h_out_shape = (2, F * 4 * T, 1, 1)
+    rng = np.random.default_rng(31)
+    cal = bl.array_tensor((A, F, 1, 2), dtype=bl.cf64, device=bl.cpu)
+    fill(cal, rng.random((A, F, 1, 2)) + 1j * rng.random((A, F, 1, 2)))
+    config_b = modeb_config(b_in_shape, b_out_shape, cal, 1, False, False)
+    config_b["beamformer_incoherent_beam"] = False
+    config_gather = {"axis": 2, "multiplier": 4}
+    config_h = {"input_shape": h_in_shape, "output_shape": h_out_shape,
+                "polarizer_convert_to_circular": True,
+                "detector_integration_size": 1, "detector_number_of_output_polarizations": 1}
+
+    @bl.runner
+    class Pipeline:
+        def __init__(self):
+            self.input.dut = bl.tensor(1, dtype=bl.f64, device=bl.cpu)
+            self.input.date = bl.tensor(1, dtype=bl.f64, device=bl.cpu)
+            self.input.buffer = bl.array_tensor(b_in_shape, dtype=bl.cf32)
+            self.output.buffer = bl.array_tensor(h_out_shape, dtype=bl.f32)
+            inp = (self.input.dut, self.input.date, self.input.buffer)
+            self.module.mode_b = bl.module(bl.modeb, config_b, inp, telescope=bl.ata)
+            self.module.gather = bl.module(bl.gather, config_gather, self.module.mode_b.get_output(), ot=bl.cf32)
+            self.module.mode_h = bl.module(bl.modeh, config_h, self.module.gather.get_output(), ot=bl.f32)
+
+        def transfer_in(self, dut, date, buffer):
+            self.copy(self.input.dut, dut)
+            self.copy(self.input.date, date)
+            self.copy(self.input.buffer, buffer)
+
+        def transfer_out(self, buffer):
+            self.copy(self.output.buffer, self.module.mode_h.get_output())
+            self.copy(buffer, self.output.buffer)
+
+    host_dut = bl.tensor(1, dtype=bl.f64, device=bl.cpu)
+    host_date = bl.tensor(1, dtype=bl.f64, device=bl.cpu)
+    host_in = bl.array_tensor(b_in_shape, dtype=bl.cf32, device=bl.cpu)
+    host_out = bl.array_tensor(h_out_shape, dtype=bl.f32, device=bl.cpu)
+    host_date.as_numpy()[0] = (1649366473.0 / 86400) + 2440587.5
+    host_dut.as_numpy()[0] = 0.0
+    x = crand(rng, b_in_shape)
+    fill(host_in, x)
+    p = Pipeline()
+    counts = {"in": 0, "out": 0, "outputs": 0}
+
+    def input_callback():
+        counts["in"] += 1
+        p.transfer_in(host_dut, host_date, host_in)
+        return bl.result.success
+
+    def output_callback():
+        counts["out"] += 1
+        p.transfer_out(host_out)
+        return bl.result.success
+
+    def callback(input_id, output_id, did_output):
+        if did_output:
+            counts["outputs"] += 1
+        return bl.result.success
+
+    for _ in range(64):                        # bounded: four enqueues give one output
+        if counts["outputs"] >= 1:
+            break
+        p.enqueue(input_callback, output_callback, counts["in"], counts["out"])
+        p.dequeue(callback)
+    assert counts["outputs"] == 1
+    assert counts["in"] >= 4                   # identical blocks: any four give the same output
+    p.synchronize()
+    phasor_mod = [m for m in p.runner._keep if m.name() == "Phasor"][0]
+    hd = bl.delay_tensor((2, A), dtype=bl.f64, device=bl.cpu)
+    bl.copy(hd, phasor_mod.get_delays())
+    w = O.phasor_ata(hd.as_numpy(), cal.as_numpy(), 6500.125e6, 0.5e6, 1.024e9, 352)
+    beams = O.mode_b(x, w)                                   # [2, F, T, 2]
+    ref = O.mode_h(np.tile(beams, (1, 1, 4, 1)), convert_to_circular=True)
+    assert ref.shape == h_out_shape
     assert O.norm_rel_err(host_out.as_numpy(), ref) <= TOL
