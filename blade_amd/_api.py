@@ -48,7 +48,8 @@ _telescope_lst = ["ata", "vla", "meerkat", "generic"]
 _device_lst = ["cpu", "cuda", "unified"]
 _types_lst = ["f16", "f32", "f64", "i8", "i16", "i32", "i64", "u8", "u16", "u32", "u64",
               "cf16", "cf32", "cf64", "ci8", "ci16", "ci32", "ci64", "cu8", "cu16", "cu32", "cu64"]
-_modules_lst = ["beamformer", "cast", "channelizer", "detector", "gather", "permutation",
+_modules_lst = ["beamformer", "cast", "channelizer", "detector", "duplicate", "gather", "guppi_reader",
+                "guppi_writer", "permutation",
                 "phasor", "polarizer", "modeh", "modeb", "batchbeamformer"]
 
 for _n in _telescope_lst + _device_lst + _types_lst + _modules_lst:
@@ -167,6 +168,15 @@ def module(name, config, input, it=None, ot=None, telescope=None):
             raise AttributeError(f"The module '{name.value}' only supports these telescopes: {', '.join(tels)}.")
     elif telescope.value != "generic":
         raise AttributeError(f"The module '{name.value}' only supports generic telescopes (telescope=bl.generic).")
+    parts = name.value.split("_")
+    taint = parts[1] if len(parts) > 1 else None
+    if any(s.startswith("taint_") for s in dir(caller)):
+        taints = [s[6:] for s in dir(caller) if s.startswith("taint_")]
+        if taint not in taints:
+            raise AttributeError(f"The module '{name.value}' only supports these taints: {', '.join(taints)}.")
+        caller = getattr(caller, f"taint_{taint}")
+    elif taint:
+        raise AttributeError(f"The module '{name.value}' doesn't support taints.")
     if any(s.startswith("in_") for s in dir(caller)):
         ins = [s[3:] for s in dir(caller) if s.startswith("in_")]
         if it.value not in ins:

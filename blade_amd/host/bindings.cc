@@ -290,6 +290,88 @@ void bind_gather(py::module_& m, const char* in_name, const char* out_name) {
 }
 
 template <typename IT, typename OT>
+void bind_duplicate(py::module_& m, const char* in_name, const char* out_name) {
+    using M = Modules::Duplicate<IT, OT>;
+    auto mm = m.def_submodule(in_name).def_submodule(out_name);
+    py::class_<M, Module, std::shared_ptr<M>> cls(mm, "mod");
+    py::class_<typename M::Config>(mm, "config").def(py::init([]() { return typename M::Config{}; }));
+    py::class_<typename M::Input>(mm, "input")
+        .def(py::init([](const ArrayTensor<Device::CUDA, IT>& buf) { return typename M::Input{buf}; }),
+             "buf"_a, py::keep_alive<0, 1>());
+    cls.def(py::init<const typename M::Config&, const typename M::Input&, const Stream&>(), "config"_a,
+            "input"_a, "stream"_a, py::keep_alive<1, 3>());
+    bind_process<M>(cls);
+    cls.def("get_input", &M::getInputBuffer, py::return_value_policy::reference_internal)
+        .def("get_output", &M::getOutputBuffer, py::return_value_policy::reference_internal);
+}
+
+// GUPPI reader/writer (reference python/blade/ext_guppi.cc:11-100): the
+// reference's class names taint_reader.type_ci8 / taint_writer.type_cf32
+// and, for bl.module, taint_reader.out_ci8.mod / taint_writer.in_cf32.mod.
+template <typename OT>
+void bind_guppi_reader(py::module_& taint, const char* out_name, const char* type_name) {
+    using M = Modules::Guppi::Reader<OT>;
+    auto mm = taint.def_submodule(out_name);
+    py::class_<M, Module, std::shared_ptr<M>> cls(mm, "mod");
+    py::class_<typename M::Config>(mm, "config")
+        .def(py::init([](const std::string& path, U64 nt, U64 nf, U64 na, U64 bs) {
+                 return typename M::Config{path, nt, nf, na, bs};
+             }),
+             "filepath"_a, "step_number_of_time_samples"_a, "step_number_of_frequency_channels"_a,
+             "step_number_of_aspects"_a, "block_size"_a = 512);
+    py::class_<typename M::Input>(mm, "input").def(py::init([]() { return typename M::Input{}; }));
+    cls.def(py::init<const typename M::Config&, const typename M::Input&, const Stream&>(), "config"_a,
+            "input"_a, "stream"_a)
+        .def(py::init([](const typename M::Config& c, const typename M::Input& i) {
+                 return std::make_shared<M>(c, i);
+             }),
+             "config"_a, "input"_a)
+        .def("process", [](M& m, U64 counter) { return m.process(counter); }, "counter"_a = 0)
+        .def("get_config", &M::getConfig, py::return_value_policy::reference_internal)
+        .def("get_step_output_buffer", &M::getStepOutputBuffer, py::return_value_policy::reference_internal)
+        .def("get_step_output_julian_date", &M::getStepOutputJulianDate, py::return_value_policy::reference_internal)
+        .def("get_step_output_dut", &M::getStepOutputDut1, py::return_value_policy::reference_internal)
+        .def("get_total_output_buffer_shape", &M::getTotalOutputBufferShape)
+        .def("get_step_output_buffer_shape", &M::getStepOutputBufferShape)
+        .def("get_number_of_steps", &M::getNumberOfSteps)
+        .def("get_total_bandwidth", &M::getTotalBandwidth)
+        .def("get_channel_bandwidth", &M::getChannelBandwidth)
+        .def("get_channel_start_index", &M::getChannelStartIndex)
+        .def("get_observation_frequency", &M::getObservationFrequency)
+        .def("__repr__", [](const M&) { return std::string("GuppiReader()"); });
+    taint.attr(type_name) = cls;
+}
+
+template <typename IT>
+void bind_guppi_writer(py::module_& taint, const char* in_name, const char* type_name) {
+    using M = Modules::Guppi::Writer<IT>;
+    auto mm = taint.def_submodule(in_name);
+    py::class_<M, Module, std::shared_ptr<M>> cls(mm, "mod");
+    py::class_<typename M::Config>(mm, "config")
+        .def(py::init([](const std::string& path, bool dio, U64 batches, U64 bs) {
+                 return typename M::Config{path, dio, batches, bs};
+             }),
+             "filepath"_a, "directio"_a, "input_frequency_batches"_a, "block_size"_a = 512);
+    py::class_<typename M::Input>(mm, "input")
+        .def(py::init([](const ArrayTensor<Device::CPU, IT>& buf) { return typename M::Input{buf}; }),
+             "buffer"_a, py::keep_alive<0, 1>());
+    cls.def(py::init<const typename M::Config&, const typename M::Input&, const Stream&>(), "config"_a,
+            "input"_a, "stream"_a, py::keep_alive<1, 3>())
+        .def(py::init([](const typename M::Config& c, const typename M::Input& i) {
+                 return std::make_shared<M>(c, i);
+             }),
+             "config"_a, "input"_a, py::keep_alive<1, 3>())
+        .def("process", [](M& m, U64 counter) { return m.process(counter); }, "counter"_a = 0)
+        .def("get_config", &M::getConfig, py::return_value_policy::reference_internal)
+        .def("get_input", &M::getInputBuffer, py::return_value_policy::reference_internal)
+        .def("header_put", [](M& m, const std::string& k, F64 v) { m.headerPut(k, v); })
+        .def("header_put", [](M& m, const std::string& k, I64 v) { m.headerPut(k, v); })
+        .def("header_put", [](M& m, const std::string& k, const std::string& v) { m.headerPut(k, v); })
+        .def("__repr__", [](const M&) { return std::string("GuppiWriter()"); });
+    taint.attr(type_name) = cls;
+}
+
+template <typename IT, typename OT>
 void bind_permutation(py::module_& m, const char* in_name, const char* out_name) {
     using M = Modules::Permutation<IT, OT>;
     auto mm = m.def_submodule(in_name).def_submodule(out_name);
@@ -577,6 +659,19 @@ PYBIND11_MODULE(_blade_ext, m) {
     bind_gather<CF16, CF16>(ga, "in_cf16", "out_cf16");
     bind_gather<CF32, CF32>(ga, "in_cf32", "out_cf32");
     bind_gather<F32, F32>(ga, "in_f32", "out_f32");
+
+    auto du = m.def_submodule("_duplicate_impl");
+    bind_duplicate<CI8, CI8>(du, "in_ci8", "out_ci8");
+    bind_duplicate<CF16, CF16>(du, "in_cf16", "out_cf16");
+    bind_duplicate<CF32, CF32>(du, "in_cf32", "out_cf32");
+    bind_duplicate<F32, F32>(du, "in_f32", "out_f32");
+
+    auto gu = m.def_submodule("_guppi_impl");
+    auto gr = gu.def_submodule("taint_reader");
+    bind_guppi_reader<CI8>(gr, "out_ci8", "type_ci8");
+    auto gw = gu.def_submodule("taint_writer");
+    bind_guppi_writer<CF16>(gw, "in_cf16", "type_cf16");
+    bind_guppi_writer<CF32>(gw, "in_cf32", "type_cf32");
 
     auto pe = m.def_submodule("_permutation_impl");
     bind_permutation<CI8, CI8>(pe, "in_ci8", "out_ci8");

@@ -351,6 +351,38 @@ template class BLADE_API Cast<CF16, F32>;
 template class BLADE_API Cast<CF16, CF32>;
 template class BLADE_API Cast<CF16, CF16>;
 
+// ----------------------------------------------------------------- Duplicate
+// (reference src/modules/duplicate/base.cc:12-42)
+
+#undef BL_LOG_DOMAIN
+#define BL_LOG_DOMAIN "M::DUPLICATE"
+
+template <typename IT, typename OT>
+Duplicate<IT, OT>::Duplicate(const Config& config, const Input& input, const Stream&)
+    : config(config), input(input) {
+    if constexpr (!std::is_same<IT, OT>::value) {
+        BL_FATAL("Input ({}) and output ({}) types aren't the same. Casting isn't supported by Duplicate yet.",
+                 TypeInfo<IT>::name, TypeInfo<OT>::name);
+        throw Result::ERROR;
+    }
+    output.buf = ArrayTensor<Device::CUDA, OT>(input.buf.shape());
+    BL_INFO("Type: {} -> {}", TypeInfo<IT>::name, TypeInfo<OT>::name);
+    BL_INFO("Shape: {} -> {}", getInputBuffer().shape(), getOutputBuffer().shape());
+}
+
+template <typename IT, typename OT>
+Result Duplicate<IT, OT>::process(const U64&, const Stream& stream) {
+    // One row of all the bytes, one step: the Gather copy kernel as a plain copy.
+    BL_ABI_CHECK(blade_gather(input.buf.data(), output.buf.data(), 1, input.buf.size() * sizeof(IT), 1, 0,
+                              abi(stream)));
+    return Result::SUCCESS;
+}
+
+template class BLADE_API Duplicate<CI8, CI8>;
+template class BLADE_API Duplicate<CF16, CF16>;
+template class BLADE_API Duplicate<CF32, CF32>;
+template class BLADE_API Duplicate<F32, F32>;
+
 // -------------------------------------------------------------------- Gather
 // (reference src/modules/gather/base.cc:12-116)
 

@@ -579,3 +579,39 @@ def test_mode_bh_pipeline_enqueue_dequeue():
     ref = O.mode_h(np.tile(beams, (1, 1, 4, 1)), convert_to_circular=True)
     assert ref.shape == h_out_shape
     assert O.norm_rel_err(host_out.as_numpy(), ref) <= TOL
+
+
+# ------------------------------------------------------------ duplicate.py
+
+@bl.runner
+class DuplicatePipeline:
+    def __init__(self, shape, config, dtype):
+        self.input.buf = bl.array_tensor(shape, dtype=dtype)
+        self.output.buf = bl.array_tensor(shape, dtype=dtype)
+        self.module.duplicate = bl.module(bl.duplicate, config, self.input.buf, it=dtype, ot=dtype)
+
+    def transfer_in(self, buf):
+        self.copy(self.input.buf, buf)
+
+    def transfer_out(self, buf):
+        self.copy(self.output.buf, self.module.duplicate.get_output())
+        self.copy(buf, self.output.buf)
+
+
+@pytest.mark.parametrize("dtype", ["cf32", "ci8", "f32"])
+def test_duplicate_module(dtype):
+    # tests/modules/duplicate/duplicate.py: a copy, checked bit for bit.
+    shape = (2, 192, 8750, 2)
+    rng = np.random.default_rng(41)
+    t = getattr(bl, dtype)
+    host_in = bl.array_tensor(shape, dtype=t, device=bl.cpu)
+    host_out = bl.array_tensor(shape, dtype=t, device=bl.cpu)
+    if dtype == "ci8":
+        fill(host_in, O.ci8_gaussian(shape, rng=rng))
+    elif dtype == "cf32":
+        fill(host_in, crand(rng, shape))
+    else:
+        fill(host_in, rng.random(shape).astype(np.float32))
+    p = DuplicatePipeline(shape, {}, t)
+    p(host_in, host_out)
+    assert np.array_equal(host_out.as_numpy(), host_in.as_numpy())
