@@ -696,6 +696,73 @@ __device__ __forceinline__ void fft16_mids(float2* p0, float2* p1, int tl, const
     }
 }
 
+// The last radix-16 pass for C = 16^k * 2 kept in registers: butterfly j
+// (Ns = NS) reads LDS and its 16 results stay in a[] and b[], at positions
+// (j / NS) * 16 NS + (j mod NS) + r NS.  With lane l of wave w taking
+// j = 32 w + (l mod 32) + (l >= 32 ? C/32 : 0), the final radix-2 pass's
+// partners (positions p and p + C/2) sit in register r of lanes l and l ^ 32.
+template <int C, int NS>
+__device__ __forceinline__ void fft16_mid_regs(const float2* __restrict__ p0, const float2* __restrict__ p1,
+                                               int j, const float2* __restrict__ twl, float2 (&a)[16],
+                                               float2 (&b)[16]) {
+    constexpr int STRIDE = C / 16;
+    float2 tb[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) tb[e] = twl[e * NS + (j & (NS - 1))];
+    const int rd = lpad(j);
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        a[r] = p0[rd + r * (STRIDE + STRIDE / 16)];
+        b[r] = p1[rd + r * (STRIDE + STRIDE / 16)];
+    }
+    float2 w[16];
+    twiddle_powers<16>(tb, w);
+#pragma unroll
+    for (int r = 1; r < 16; r++) {
+        a[r] = cmul(a[r], w[r]);
+        b[r] = cmul(b[r], w[r]);
+    }
+    dft16(a);
+    dft16(b);
+}
+
+// One polarization's plane of fft16_mid_regs.
+template <int C, int NS>
+__device__ __forceinline__ void fft16_mid_regs1(const float2* __restrict__ pl, int j, const float2* __restrict__ twl,
+                                                float2 (&a)[16]) {
+    constexpr int STRIDE = C / 16;
+    float2 tb[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) tb[e] = twl[e * NS + (j & (NS - 1))];
+    const int rd = lpad(j);
+#pragma unroll
+    for (int r = 0; r < 16; r++) a[r] = pl[rd + r * (STRIDE + STRIDE / 16)];
+    float2 w[16];
+    twiddle_powers<16>(tb, w);
+#pragma unroll
+    for (int r = 1; r < 16; r++) a[r] = cmul(a[r], w[r]);
+    dft16(a);
+}
+
+// W_32^r = exp(-j 2 pi r / 32), r < 16, from cos(k pi / 16).
+__device__ __forceinline__ float2 w32(int r) {
+    constexpr float c1 = 0.98078528040323044913f, c2 = 0.92387953251128675613f;
+    constexpr float c3 = 0.83146961230254523708f, c4 = 0.70710678118654752440f;
+    constexpr float c5 = 0.55557023301960222474f, c6 = 0.38268343236508977173f;
+    constexpr float c7 = 0.19509032201612826785f;
+    constexpr float re[16] = {1.f, c1, c2, c3, c4, c5, c6, c7, 0.f, -c7, -c6, -c5, -c4, -c3, -c2, -c1};
+    constexpr float im[16] = {0.f, -c7, -c6, -c5, -c4, -c3, -c2, -c1, -1.f, -c1, -c2, -c3, -c4, -c5, -c6, -c7};
+    return make_float2(re[r], im[r]);
+}
+
+// x at lanes l < 32 holds u, at lanes l >= 32 holds v (same register):
+// returns u and v in every lane.
+__device__ __forceinline__ void halves(float x, float& u, float& v) {
+    const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    u = __uint_as_float(p[0]);
+    v = __uint_as_float(p[1]);
+}
+
 template <int LC, int NT, int MODE, bool NTS>
 __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__ in, void* __restrict__ outv,
                                                       uint64_t S, uint64_t n_items, int N, int accumulate,
@@ -711,6 +778,9 @@ __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__
     constexpr int NSL = C / R;                         // Ns (and stride) of the last pass
     constexpr int PLANE = C + C / 16;
     constexpr int NPOL = MODE == 0 ? 1 : MODE;
+    // Final radix-2 pass across lane halves (whole waves per spectrum); the
+    // full-Stokes sums need both polarizations at once and keep the LDS pass.
+    constexpr bool SW = RL == 2 && TPS % 64 == 0 && NMID >= 1 && MODE != 4;
     extern __shared__ float2 lds[];
     const int slot = threadIdx.x / TPS;
     const int tl = threadIdx.x % TPS;
@@ -734,6 +804,17 @@ __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__
     float2 tb[4];
 #pragma unroll
     for (int e = 0; e < 4; e++) tb[e] = tw[(tl << e) & (C - 1)];
+    // SW: lane l of wave w handles jlow = 32 w + (l mod 32) in the upper
+    // (l >= 32) or lower half; its final-pass twiddle base is W_C^jlow.
+    const bool hi = (tl & 63) >= 32;
+    const int jlow = (tl >> 6) * 32 + (tl & 31);
+    const float2 tsw = SW ? tw[jlow] : make_float2(1.f, 0.f);
+    // Output channel (fftshifted) of result i: lane base + per-result offset.
+    const int ch0 = SW ? jlow + (hi ? 0 : C / 2) : tl;
+    auto kofs = [](int i) -> int {
+        if constexpr (SW) return i * (C / 32);
+        else return ((i / R) * TPS + (i % R) * NSL + (C >> 1)) & (C - 1);
+    };
 
     const uint64_t per_round = (uint64_t)gridDim.x * SPW;
     const uint64_t rounds = (n_items + per_round - 1) / per_round;
@@ -804,9 +885,46 @@ __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__
                 }
             }
             __syncthreads();
+            float2 a[16], b[16];
+            if constexpr (SW) {
+                // ---- last radix-16 pass in registers, then radix 2 across
+                // lane halves: X(p) = u + W v in the lower half, X(p + C/2) =
+                // u - W v in the upper, W = W_C^(jlow + r C/32).
+                fft16_mids<C, NMID - 1, 0>(p0, p1, tl, twl);
+                constexpr int NSR = 1 << (4 * NMID);
+                const int jr = jlow + (hi ? C / 32 : 0);
+                const float2* const twr = twl + fft16_twl_offset(NMID - 1);
+                const float sg = hi ? -1.f : 1.f;
+                auto radix2 = [&](float2& x, const float2 wr) {
+                    float2 u, v;
+                    halves(x.x, u.x, v.x);
+                    halves(x.y, u.y, v.y);
+                    const float2 t = cmul(wr, v);
+                    x = make_float2(u.x + sg * t.x, u.y + sg * t.y);
+                };
+                if constexpr (MODE == 1) {
+                    // Stokes I: one polarization at a time (half the registers).
+#pragma unroll
+                    for (int q = 0; q < 2; q++) {
+                        fft16_mid_regs1<C, NSR>(q == 0 ? p0 : p1, jr, twr, a);
+#pragma unroll
+                        for (int r = 0; r < 16; r++) {
+                            radix2(a[r], cmul(tsw, w32(r)));
+                            acc[r][0] += a[r].x * a[r].x + a[r].y * a[r].y;
+                        }
+                    }
+                } else {
+                    fft16_mid_regs<C, NSR>(p0, p1, jr, twr, a, b);
+#pragma unroll
+                    for (int r = 0; r < 16; r++) {
+                        const float2 wr = cmul(tsw, w32(r));
+                        radix2(a[r], wr);
+                        radix2(b[r], wr);
+                    }
+                }
+            } else {
             fft16_mids<C, NMID, 0>(p0, p1, tl, twl);
             // ---- last pass: radix R, LDS -> output (Ns = C / R)
-            float2 a[16], b[16];
             const int rd = lpad(tl);
 #pragma unroll
             for (int g = 0; g < G; g++)
@@ -854,20 +972,22 @@ __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__
                     for (int r = 0; r < R; r++) { a[g * R + r] = xa[r]; b[g * R + r] = xb[r]; }
                 }
             }
-            // Channel k = tl + K with K = g C/16 + r C/R; fftshifted
-            // ko = tl + ((K + C/2) mod C), the bracket a multiple of C/16.
+            }
+            // Channel of result i (fftshifted): ch0 + kofs(i).  Without SW,
+            // k = tl + K with K = g C/16 + r C/R and ko = tl + ((K + C/2) mod
+            // C), the bracket a multiple of C/16.
             // Idle lanes store into the sink (same store count on every lane
             // and path, so waits on the next spectrum's loads stay exact).
             if constexpr (MODE == 0) {
                 const uint64_t af = item / S, si = item - af * S;
-                float4* const lb = valid ? static_cast<float4*>(outv) + (af * C * S + si) + (uint64_t)tl * S
+                float4* const lb = valid ? static_cast<float4*>(outv) + (af * C * S + si) + (uint64_t)ch0 * S
                                          : sink + threadIdx.x;
                 const uint64_t step = valid ? S : 0;
 #pragma unroll
                 for (int g = 0; g < G; g++)
 #pragma unroll
                     for (int r = 0; r < R; r++) {
-                        const int kb = (g * TPS + r * NSL + (C >> 1)) & (C - 1);
+                        const int kb = kofs(g * R + r);
                         const float2 X = a[g * R + r], Y = b[g * R + r];
                         float4* const o = lb + (uint64_t)kb * step;
                         if constexpr (NTS) {
@@ -886,6 +1006,7 @@ __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__
             } else {
 #pragma unroll
                 for (int i = 0; i < 16; i++) {
+                    if constexpr (SW && MODE == 1) break;     // summed in the register pass
                     const float2 X = a[i], Y = b[i];
                     if constexpr (MODE == 1) {
                         acc[i][0] += X.x * X.x + X.y * X.y + Y.x * Y.x + Y.y * Y.y;
@@ -900,9 +1021,12 @@ __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__
                 if (flush) {
                     const int grp = (N <= (int)S) ? s / N : 0;
                     const unsigned gn = (unsigned)groups * NPOL;
-                    float* const lb = valid ? static_cast<float*>(outv) + (item * C + tl) * gn + grp * NPOL
+                    float* const lb = valid ? static_cast<float*>(outv) + (item * C + ch0) * gn + grp * NPOL
                                             : reinterpret_cast<float*>(sink + threadIdx.x);
-                    const unsigned step = valid ? gn : 0;
+                    unsigned step = valid ? gn : 0;
+                    // Opaque: keeps the 16 output addresses from being hoisted
+                    // out of the spectrum loop (they would pin 32 registers).
+                    asm volatile("" : "+v"(step));
                     // All earlier sums are read before any store (idle lanes'
                     // addresses coincide, so the compiler cannot interleave).
                     // The four full-Stokes sums of a channel are one 16-byte
@@ -913,7 +1037,7 @@ __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__
                         for (int g = 0; g < G; g++)
 #pragma unroll
                             for (int r = 0; r < R; r++) {
-                                const int kb = (g * TPS + r * NSL + (C >> 1)) & (C - 1);
+                                const int kb = kofs(g * R + r);
                                 const float* const o = lb + (uint64_t)((unsigned)kb * step);
                                 if constexpr (NPOL == 4) {
                                     const float4 v = *reinterpret_cast<const float4*>(o);
@@ -934,7 +1058,7 @@ __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__
                     for (int g = 0; g < G; g++)
 #pragma unroll
                         for (int r = 0; r < R; r++) {
-                            const int kb = (g * TPS + r * NSL + (C >> 1)) & (C - 1);
+                            const int kb = kofs(g * R + r);
                             float* const o = lb + (uint64_t)((unsigned)kb * step);
                             const int i = g * R + r;
                             if constexpr (NPOL == 4)
