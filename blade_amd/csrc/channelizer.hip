@@ -763,7 +763,10 @@ __device__ __forceinline__ void halves(float x, float& u, float& v) {
     v = __uint_as_float(p[1]);
 }
 
-template <int LC, int NT, int MODE, bool NTS>
+// SM (MODE 0 stores): 0 strided, 1 nontemporal with one spectrum per
+// (antenna, channel), 2 transposed through LDS so that the workgroup's SPW
+// consecutive spectra of one (antenna, channel) land as whole lines.
+template <int LC, int NT, int MODE, int SM>
 __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__ in, void* __restrict__ outv,
                                                       uint64_t S, uint64_t n_items, int N, int accumulate,
                                                       const float2* __restrict__ tw, float4* __restrict__ sink) {
@@ -978,7 +981,41 @@ __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__
             // C), the bracket a multiple of C/16.
             // Idle lanes store into the sink (same store count on every lane
             // and path, so waits on the next spectrum's loads stay exact).
-            if constexpr (MODE == 0) {
+            if constexpr (MODE == 0 && SM == 2) {
+                // S is a multiple of SPW: the workgroup's spectra are
+                // si0 .. si0 + SPW - 1 of one (antenna, channel), whose outputs
+                // interleave at (ko S + si).  Stage them [ko][spectrum] in LDS
+                // (one float4 of padding per 16) and store runs of SPW.
+                __syncthreads();                    // the last pass's LDS reads are done
+                float4* const L = reinterpret_cast<float4*>(lds);
+                int base = ch0 * SPW + slot;
+                asm volatile("" : "+v"(base));      // not hoisted: see below
+#pragma unroll
+                for (int i = 0; i < 16; i++) {
+                    const int idx = base + kofs(i) * SPW;
+                    L[idx + (idx >> 4)] = make_float4(a[i].x, a[i].y, b[i].x, b[i].y);
+                }
+                __syncthreads();
+                // Thread t stores entries t + i NT, i < 16: spectrum t mod SPW,
+                // channel t / SPW + i NT / SPW (NT is a multiple of SPW).
+                const bool wg_valid = first < n_items;
+                const uint64_t af0 = first / S, si0 = first - af0 * S;
+                const int tx = threadIdx.x;
+                float4* const lb = wg_valid ? static_cast<float4*>(outv) + af0 * C * S + si0 +
+                                                  (uint64_t)(tx / SPW) * S + (tx % SPW)
+                                            : sink + tx;
+                const uint64_t step = wg_valid ? (uint64_t)(NT / SPW) * S : 0;
+                const int rb = tx + (tx >> 4);
+#pragma unroll
+                for (int i = 0; i < 16; i++) {
+                    const float4 v = L[rb + i * (NT + NT / 16)];
+                    float* const of = reinterpret_cast<float*>(lb + i * step);
+                    __builtin_nontemporal_store(v.x, of);
+                    __builtin_nontemporal_store(v.y, of + 1);
+                    __builtin_nontemporal_store(v.z, of + 2);
+                    __builtin_nontemporal_store(v.w, of + 3);
+                }
+            } else if constexpr (MODE == 0) {
                 const uint64_t af = item / S, si = item - af * S;
                 float4* const lb = valid ? static_cast<float4*>(outv) + (af * C * S + si) + (uint64_t)ch0 * S
                                          : sink + threadIdx.x;
@@ -990,7 +1027,7 @@ __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__
                         const int kb = kofs(g * R + r);
                         const float2 X = a[g * R + r], Y = b[g * R + r];
                         float4* const o = lb + (uint64_t)kb * step;
-                        if constexpr (NTS) {
+                        if constexpr (SM == 1) {
                             // Component-wise: merged into one nontemporal
                             // global_store_dwordx4 (a vector operand here is
                             // lowered through private memory).
@@ -1081,14 +1118,14 @@ __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__
 
 // Launch of fft16_kernel for C = 2^LC; NT threads use NT * 272 bytes of LDS
 // (one 512-thread workgroup per CU).
-template <int LC, int NT, int MODE, bool NTS>
+template <int LC, int NT, int MODE, int SM>
 static int launch_fft16_t(const float2* in, void* out, uint64_t S, uint64_t n_items, int N, int accumulate,
                           const float2* tw, hipStream_t s) {
     constexpr int C = 1 << LC;
     constexpr int SPW = NT / (C / 16);
     constexpr int NMID = LC / 4 - ((LC % 4) ? 1 : 2);
     constexpr size_t lds = sizeof(float2) * (2 * (size_t)(C + C / 16) * SPW + fft16_twl_offset(NMID));
-    const void* fn = reinterpret_cast<const void*>(&fft16_kernel<LC, NT, MODE, NTS>);
+    const void* fn = reinterpret_cast<const void*>(&fft16_kernel<LC, NT, MODE, SM>);
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) {
@@ -1101,7 +1138,7 @@ static int launch_fft16_t(const float2* in, void* out, uint64_t S, uint64_t n_it
     const uint64_t rounds = ceil_div(nb, resident);
     const uint64_t blocks = ceil_div(nb, rounds);      // equal rounds for every workgroup
     float4* sink = reinterpret_cast<float4*>(const_cast<float2*>(tw) + C);
-    hipLaunchKernelGGL((fft16_kernel<LC, NT, MODE, NTS>), dim3((unsigned)blocks), dim3(NT), lds, s,
+    hipLaunchKernelGGL((fft16_kernel<LC, NT, MODE, SM>), dim3((unsigned)blocks), dim3(NT), lds, s,
                        reinterpret_cast<const float4*>(in), out, S, n_items, N, accumulate, tw, sink);
     return check_launch("channelizer");
 }
@@ -1112,13 +1149,21 @@ static bool use_fft16(int C) { return C >= 256 && C <= 8192 && !(C & (C - 1)); }
 template <int LC, int MODE>
 static int launch_fft16_lc(const float2* in, void* out, uint64_t S, uint64_t n_items, int N, int accumulate,
                            const float2* tw, hipStream_t s) {
-    // Nontemporal stores pay off only when each spectrum's output is
-    // contiguous (one spectrum per (antenna, channel)); with interleaved
-    // spectra they turn the partial-line writes into much slower ones.
+    // Nontemporal stores pay off only where the stores are whole lines: one
+    // spectrum per (antenna, channel), or several staged through LDS when a
+    // workgroup holds whole runs of at least four of them (S a multiple of
+    // its SPW spectra, so each row of the staged tile is 64 contiguous
+    // bytes); with two per row the LDS round trip costs more than the 32-byte
+    // pieces save, and interleaved partial lines with nontemporal stores are
+    // much slower than plain stores.
     if constexpr (MODE == 0) {
-        if (S == 1) return launch_fft16_t<LC, 512, MODE, true>(in, out, S, n_items, N, accumulate, tw, s);
+        constexpr uint64_t SPW = 512 / ((1 << LC) / 16);
+        if (S == 1) return launch_fft16_t<LC, 512, MODE, 1>(in, out, S, n_items, N, accumulate, tw, s);
+        if constexpr (SPW >= 4) {
+            if (S % SPW == 0) return launch_fft16_t<LC, 512, MODE, 2>(in, out, S, n_items, N, accumulate, tw, s);
+        }
     }
-    return launch_fft16_t<LC, 512, MODE, false>(in, out, S, n_items, N, accumulate, tw, s);
+    return launch_fft16_t<LC, 512, MODE, 0>(in, out, S, n_items, N, accumulate, tw, s);
 }
 
 constexpr uint64_t NT_MAX_SPECTRA = 32;    // spectra per workgroup at C = 256, 512 threads

@@ -188,6 +188,16 @@ def test_channelizer_power_of_two_rates(C):
     assert O.norm_rel_err(y, O.channelizer(x, C)) <= TOL
 
 
+@pytest.mark.parametrize("C", [256, 1024, 2048, 4096])
+def test_channelizer_many_spectra_per_channel(C):
+    # T = 16384: 16384 / C spectra per (antenna, channel), a multiple of the
+    # spectra a workgroup holds, which stages them in LDS before the store.
+    rng = np.random.default_rng(C + 2)
+    x = crand(rng, 2, 3, 16384, 2)
+    y = host(ops.channelizer(dev(x), C))
+    assert O.norm_rel_err(y, O.channelizer(x, C)) <= TOL
+
+
 @pytest.mark.parametrize("C", POW2_RATES)
 @pytest.mark.parametrize("npol", [1, 4])
 def test_channelizer_detector_power_of_two_rates(C, npol):
