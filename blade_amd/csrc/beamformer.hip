@@ -95,50 +95,27 @@ struct BfArgs {
 };
 
 // Packs the phasors of one (f, beam group) into the A-operand layout the
-// beamformer copies verbatim into LDS, followed by |w0|^2[p][a] (256 floats)
-// and the split-FP16 unscale factors (2 floats).
+// beamformer reads from LDS, followed by |w0|^2[p][a] (256 floats) and the
+// split-FP16 unscale factors (2 floats).
 //   FP32       [m][p][step][g][row] float, step = 2 * antenna group + component
 //   split FP16 [m][p][step][hl][lane] halfx8 (8 k values = 4 antennas x re/im),
 //              components scaled by a per-pol power of two so that all are < 1,
 //              hi = fp16(w), lo = fp16(w - hi).
-// One workgroup per (pol p, beam tile m) of one (f, beam group): grid
-// (MW * 2, F, groups).  The split-FP16 scale is shared by the whole beam group
-// of a pol, so that path stages all MW * 8 beams of the pol for the maximum.
 // G3 selects the three-product FP32 layout (16-beam tiles, see bf_body).
-template <int MW, bool F16X2, bool G3 = (!F16X2 && MW == 2)>
-__device__ __forceinline__ void bf_pack_body(const BfArgs& a, const int mp, const int f, const int gz,
-                                             const int groups, float2* s_w, unsigned int& s_max) {
+//
+// bf_pack_fill writes the (beam tile m, pol p) part from staged phasors `sw`
+// ([SB][A] float2 of pol p: the 8 beams of tile m for FP32, all MW * 8 beams
+// of the group for split FP16, whose scale is shared by the group's pol).
+template <int MW, bool F16X2, bool G3>
+__device__ __forceinline__ void bf_pack_fill(const BfArgs& a, const int m, const int p, const float2* sw,
+                                             const float scale, unsigned char* slab) {
     constexpr int AK = F16X2 ? 4 * KG : KG;
-    constexpr int NB = MW * (TILE / 2);              // beams of one group
-    constexpr int SB = F16X2 ? NB : TILE / 2;        // beams staged in LDS (s_w[SB][A], pol p)
-    const int p = mp & 1;
-    const int m = mp >> 1;
     const int A = a.A;
     const int nsteps = a.nsteps;
-    const int b0 = gz * NB + (F16X2 ? 0 : m * (TILE / 2));   // first staged beam
-    unsigned char* slab = a.pack + ((size_t)f * groups + gz) * a.slab_bytes;
-    float* tail = reinterpret_cast<float*>(slab + a.wbytes);
-    if (threadIdx.x == 0) s_max = 0u;
-    float mx = 0.f;
-    for (int e = threadIdx.x; e < SB * A; e += blockDim.x) {
-        const int bl_ = e / A, an = e - bl_ * A, b = b0 + bl_;
-        float2 w = make_float2(0.f, 0.f);
-        if (b < a.B) w = a.w[(((size_t)b * A + an) * a.F + f) * 2 + p];
-        s_w[e] = w;
-        mx = fmaxf(mx, fmaxf(fabsf(w.x), fabsf(w.y)));
-    }
-    __syncthreads();
-    // beam (within the staged rows) of A-operand row `row` of tile m
-    const int row_beam0 = F16X2 ? m * (TILE / 2) : 0;
-    float unscale = 1.f;
     if constexpr (F16X2) {
-        atomicMax(&s_max, __float_as_uint(mx));
-        __syncthreads();
-        int ex = 0;
-        frexpf(__uint_as_float(s_max), &ex);
-        const float scale = ldexpf(1.0f, -ex);
-        unscale = ldexpf(1.0f, ex);
-        // [m][p][step][hl][lane] halfx8, this workgroup's (m, p) block
+        // beam (within the staged rows) of A-operand row `row` of tile m
+        const int row_beam0 = m * (TILE / 2);
+        // [m][p][step][hl][lane] halfx8, the (m, p) block
         halfx8* wh = reinterpret_cast<halfx8*>(slab) + (size_t)((m * 2 + p) * nsteps) * 128;
         for (int e = threadIdx.x; e < nsteps * 128; e += blockDim.x) {
             const int st = e >> 7, hl = (e >> 6) & 1, ln = e & 63;
@@ -150,7 +127,7 @@ __device__ __forceinline__ void bf_pack_body(const BfArgs& a, const int mp, cons
                 const int an = st * AK + 4 * kg + q;
                 float wr = 0.f, wi = 0.f;
                 if (an < A) {
-                    const float2 w = s_w[bl_ * A + an];
+                    const float2 w = sw[bl_ * A + an];
                     wr = w.x * scale;
                     wi = w.y * scale;
                 }
@@ -170,7 +147,7 @@ __device__ __forceinline__ void bf_pack_body(const BfArgs& a, const int mp, cons
         }
     } else if constexpr (G3) {
         // Three-product layout [p][unit][q][g][row] float, q = (Wr, Wi, Wr + Wi),
-        // row = beam of the 16-beam tile; this workgroup writes rows 8m .. 8m + 7.
+        // row = beam of the 16-beam tile; this part is rows 8m .. 8m + 7.
         const int nunits = nsteps / 2;
         float* wl = reinterpret_cast<float*>(slab) + (size_t)p * nunits * 3 * 64;
         for (int e = threadIdx.x; e < nunits * 3 * 32; e += blockDim.x) {
@@ -179,7 +156,7 @@ __device__ __forceinline__ void bf_pack_body(const BfArgs& a, const int mp, cons
             const int an = un * KG + kg;
             float v = 0.f;
             if (an < A) {
-                const float2 w = s_w[rl * A + an];
+                const float2 w = sw[rl * A + an];
                 v = q == 0 ? w.x : (q == 1 ? w.y : w.x + w.y);
             }
             wl[(un * 3 + q) * 64 + kg * TILE + m * (TILE / 2) + rl] = v;
@@ -193,34 +170,114 @@ __device__ __forceinline__ void bf_pack_body(const BfArgs& a, const int mp, cons
             const int bl_ = row >> 1, co = row & 1;
             float v = 0.f;
             if (an < A) {
-                const float2 w = s_w[bl_ * A + an];
+                const float2 w = sw[bl_ * A + an];
                 v = (co == 0) ? (c == 0 ? w.x : -w.y) : (c == 0 ? w.y : w.x);
             }
             wl[e] = v;
         }
     }
-    if (m == 0) {
-        // |w0|^2 of pol p (read only by the beam-group-0 workgroups, whose
-        // staged row 0 is beam 0) and the unscale of pol p.
-        if (gz == 0) {
-            for (int an = threadIdx.x; an < 128; an += blockDim.x) {
-                float v = 0.f;
-                if (an < A) {
-                    const float2 w = s_w[an];
-                    v = w.x * w.x + w.y * w.y;
-                }
-                tail[p * 128 + an] = v;
-            }
+}
+
+// |w0|^2 of pol p over the antennas (row 0 of the staged beams), read only
+// by the beam-group-0 workgroups, whose staged row 0 is beam 0.
+__device__ __forceinline__ void bf_pack_w0sq(const BfArgs& a, const float2* sw, float* w0sq) {
+    for (int an = threadIdx.x; an < 128; an += blockDim.x) {
+        float v = 0.f;
+        if (an < a.A) {
+            const float2 w = sw[an];
+            v = w.x * w.x + w.y * w.y;
         }
+        w0sq[an] = v;
+    }
+}
+
+// One (pol p, beam tile m) part of one (f, beam group), into the global
+// slab (the batched pack kernel).  The split-FP16 path stages all MW * 8
+// beams of the pol for the maximum.
+template <int MW, bool F16X2, bool G3 = (!F16X2 && MW == 2)>
+__device__ __forceinline__ void bf_pack_body(const BfArgs& a, const int mp, const int f, const int gz,
+                                             const int groups, float2* s_w, unsigned int& s_max) {
+    constexpr int NB = MW * (TILE / 2);              // beams of one group
+    constexpr int SB = F16X2 ? NB : TILE / 2;        // beams staged in LDS (s_w[SB][A], pol p)
+    const int p = mp & 1;
+    const int m = mp >> 1;
+    const int A = a.A;
+    const int b0 = gz * NB + (F16X2 ? 0 : m * (TILE / 2));   // first staged beam
+    unsigned char* slab = a.pack + ((size_t)f * groups + gz) * a.slab_bytes;
+    float* tail = reinterpret_cast<float*>(slab + a.wbytes);
+    if (threadIdx.x == 0) s_max = 0u;
+    float mx = 0.f;
+    for (int e = threadIdx.x; e < SB * A; e += blockDim.x) {
+        const int bl_ = e / A, an = e - bl_ * A, b = b0 + bl_;
+        float2 w = make_float2(0.f, 0.f);
+        if (b < a.B) w = a.w[(((size_t)b * A + an) * a.F + f) * 2 + p];
+        s_w[e] = w;
+        mx = fmaxf(mx, fmaxf(fabsf(w.x), fabsf(w.y)));
+    }
+    __syncthreads();
+    float scale = 1.f, unscale = 1.f;
+    if constexpr (F16X2) {
+        atomicMax(&s_max, __float_as_uint(mx));
+        __syncthreads();
+        int ex = 0;
+        frexpf(__uint_as_float(s_max), &ex);
+        scale = ldexpf(1.0f, -ex);
+        unscale = ldexpf(1.0f, ex);
+    }
+    bf_pack_fill<MW, F16X2, G3>(a, m, p, s_w, scale, slab);
+    if (m == 0) {
+        if (gz == 0) bf_pack_w0sq(a, s_w, tail + p * 128);
         if (threadIdx.x == 0) tail[256 + p] = unscale;
     }
 }
 
-template <int MW, bool F16X2>
-__global__ __launch_bounds__(256) void bf_pack_kernel(BfArgs a) {
-    extern __shared__ float2 s_wp[];
-    __shared__ unsigned int s_max;
-    bf_pack_body<MW, F16X2>(a, blockIdx.x, blockIdx.y, blockIdx.z, gridDim.z, s_wp, s_max);
+// The whole (f, beam group) straight into the workgroup's LDS: one round of
+// phasor loads for both pols and all MW * 8 beams (s_w: [2][MW * 8][A]),
+// then every (m, p) part.  Ends with a barrier.
+template <int MW, bool F16X2, bool G3>
+__device__ __forceinline__ void bf_pack_lds(const BfArgs& a, const int f, const int gz, float2* s_w,
+                                            unsigned int* s_max, unsigned char* dst, float* w0sq,
+                                            float* unscale) {
+    constexpr int NB = MW * (TILE / 2);
+    const int A = a.A;
+    if (threadIdx.x < 2) s_max[threadIdx.x] = 0u;
+    float mx[2] = {0.f, 0.f};
+    for (int e = threadIdx.x; e < 2 * NB * A; e += blockDim.x) {
+        const int p = e / (NB * A), r = e - p * (NB * A);
+        const int bl_ = r / A, an = r - bl_ * A, b = gz * NB + bl_;
+        float2 w = make_float2(0.f, 0.f);
+        if (b < a.B) w = a.w[(((size_t)b * A + an) * a.F + f) * 2 + p];
+        s_w[e] = w;
+        const float v = fmaxf(fabsf(w.x), fabsf(w.y));
+        mx[0] = p == 0 ? fmaxf(mx[0], v) : mx[0];
+        mx[1] = p == 1 ? fmaxf(mx[1], v) : mx[1];
+    }
+    __syncthreads();
+    float scale[2] = {1.f, 1.f}, uns[2] = {1.f, 1.f};
+    if constexpr (F16X2) {
+        atomicMax(&s_max[0], __float_as_uint(mx[0]));
+        atomicMax(&s_max[1], __float_as_uint(mx[1]));
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            int ex = 0;
+            frexpf(__uint_as_float(s_max[p]), &ex);
+            scale[p] = ldexpf(1.0f, -ex);
+            uns[p] = ldexpf(1.0f, ex);
+        }
+    }
+#pragma unroll
+    for (int mp = 0; mp < 2 * MW; mp++) {
+        const int p = mp & 1, m = mp >> 1;
+        const float2* sw = s_w + (size_t)p * NB * A + (F16X2 ? 0 : m * (TILE / 2) * A);
+        bf_pack_fill<MW, F16X2, G3>(a, m, p, sw, scale[p], dst);
+    }
+    if (gz == 0) {
+        bf_pack_w0sq(a, s_w, w0sq);
+        bf_pack_w0sq(a, s_w + (size_t)NB * A, w0sq + 128);
+    }
+    if (threadIdx.x < 2) unscale[threadIdx.x] = uns[threadIdx.x];
+    __syncthreads();
 }
 
 __device__ __forceinline__ void polarize2(float2& X, float2& Y) {
@@ -368,16 +425,26 @@ __device__ __forceinline__ void load_words(uint32_t* dst, const uint32_t* src) {
 //   split FP16 wh[m][p][step][hl][lane]      halfx8  (8 k values = 4 antennas x re/im)
 //
 // Work split: blockIdx = (t-range, f, beam group of MW tiles).  A workgroup
-// walks `chunks_per_wg` consecutive chunks of WAVES*TILE*NT samples with the
+// walks up to `chunks_per_wg` chunks of WAVES*TILE*NT samples with the
 // phasors staged once; voltage loads run D units ahead of the MFMAs through
 // a register ring that crosses chunk boundaries.
-// One work item: channel f, beam group gz (of `groups`), chunks cw0 ..
-// cw0 + chunks_per_wg - 1 of the block described by `a`.  LDS: lds_raw holds
-// the phasor A operands, s_unscale / s_w0sq the slab's tail.
-template <int NT, int MW, bool CI8, bool F16X2, int D, bool NTS, int EPI, bool G3 = (!F16X2 && MW == 2)>
+// One work item: channel f, beam group gz (of `groups`), chunks cw0,
+// cw0 + cstride, cw0 + 2 cstride, ... of the block described by `a`.  The
+// single-block kernel interleaves (cw0 = blockIdx.x, cstride = gridDim.x),
+// so the workgroups of one channel read and write adjacent pieces of each
+// antenna and beam row at the same time; with voltages coming from HBM
+// rather than the last-level cache that is ~3% faster than consecutive
+// chunk ranges.  LDS: lds_raw holds the phasor A operands, s_unscale /
+// s_w0sq the slab's tail.
+// FUSE: the workgroup packs its phasors straight into LDS (bf_pack_lds; the
+// dynamic LDS then holds the staged phasors [2][MW * 8][A] float2 after the
+// A operands) instead of copying a slab written by a pack kernel.
+template <int NT, int MW, bool CI8, bool F16X2, int D, bool NTS, int EPI, bool G3 = (!F16X2 && MW == 2),
+          bool FUSE = false>
 __device__ __forceinline__ void bf_body(const BfArgs& a, const int f, const int gz, const int cw0,
                                         const int groups, unsigned char* lds_raw,
-                                        float* s_unscale, float (*s_w0sq)[128]) {
+                                        float* s_unscale, float (*s_w0sq)[128], const int cstride = 1,
+                                        unsigned int* s_max = nullptr) {
     constexpr int AK = F16X2 ? 4 * KG : KG;       // antennas per unit
     typedef Unit<NT, CI8, F16X2> U_t;
 
@@ -391,13 +458,13 @@ __device__ __forceinline__ void bf_body(const BfArgs& a, const int f, const int 
 
     const int n_units = F16X2 ? nsteps : nsteps / 2;
     const int upad = ((n_units + D - 1) / D) * D;
-    const int nch = min(a.chunks_per_wg, a.n_tchunks - cw0);
+    const int nch = min(a.chunks_per_wg, (a.n_tchunks - cw0 + cstride - 1) / cstride);
     const int J = nch * upad;
     const size_t ant_stride = (size_t)a.F * a.T;   // complex samples between antennas
 
     typedef TimeMap<NT, epi_run<NT, EPI>()> M;
     // First sample of this lane in chunk c (runs of M::RUN samples, TimeMap).
-    auto chunk_t0 = [&](int c) { return ((cw0 + c) * WAVES + wave) * (TILE * NT) + M::RUN * i; };
+    auto chunk_t0 = [&](int c) { return ((cw0 + c * cstride) * WAVES + wave) * (TILE * NT) + M::RUN * i; };
 
     // Loads are unconditional so that the wait-count analysis keeps them in
     // flight across the loop: indices past the end re-load the last unit,
@@ -439,11 +506,15 @@ __device__ __forceinline__ void bf_body(const BfArgs& a, const int f, const int 
 #pragma unroll
     for (int d = 0; d < D; d++) load_unit(d, ring[d]);
 
-    // ---- Copy this (f, beam group)'s packed phasors into LDS. ---------------
-    // The slab was laid out by bf_pack_kernel: A operands, |w0|^2 per
-    // antenna and pol, and the power-of-two unscale of the split-FP16 path.
-    const unsigned char* slab = a.pack + ((size_t)f * groups + gz) * a.slab_bytes;
-    {
+    // ---- This (f, beam group)'s packed phasors into LDS. ---------------------
+    // A operands, |w0|^2 per antenna and pol, and the power-of-two unscale of
+    // the split-FP16 path: packed here, or copied from the slab that the
+    // batched pack kernel wrote.
+    if constexpr (FUSE) {
+        bf_pack_lds<MW, F16X2, G3>(a, f, gz, reinterpret_cast<float2*>(lds_raw + a.wbytes), s_max, lds_raw,
+                                   &s_w0sq[0][0], s_unscale);
+    } else {
+        const unsigned char* slab = a.pack + ((size_t)f * groups + gz) * a.slab_bytes;
         const uint4* src = reinterpret_cast<const uint4*>(slab);
         uint4* dst = reinterpret_cast<uint4*>(lds_raw);
         const int n16 = (int)(a.wbytes / 16);
@@ -453,8 +524,8 @@ __device__ __forceinline__ void bf_body(const BfArgs& a, const int f, const int 
             for (int e = threadIdx.x; e < 2 * 128; e += blockDim.x) (&s_w0sq[0][0])[e] = tail[e];
         }
         if (threadIdx.x < 2) s_unscale[threadIdx.x] = tail[256 + threadIdx.x];
+        __syncthreads();
     }
-    __syncthreads();
     const float unscale[2] = {s_unscale[0], s_unscale[1]};
 
     // Three-product FP32 tiles (16 beams): accumulators P1, P2, P3.
@@ -674,14 +745,15 @@ __device__ __forceinline__ void bf_body(const BfArgs& a, const int f, const int 
     }
 }
 
-// One block: blockIdx = (chunk range, f, beam group).
+// One block: blockIdx = (first chunk, f, beam group); chunks interleave.
 template <int NT, int MW, bool CI8, bool F16X2, int D, bool NTS, int EPI>
 __global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ float s_unscale[2];
     __shared__ float s_w0sq[2][128];
-    bf_body<NT, MW, CI8, F16X2, D, NTS, EPI>(a, blockIdx.y, blockIdx.z, blockIdx.x * a.chunks_per_wg,
-                                            gridDim.z, lds_raw, s_unscale, s_w0sq);
+    __shared__ unsigned int s_max[2];
+    bf_body<NT, MW, CI8, F16X2, D, NTS, EPI, (!F16X2 && MW == 2), true>(
+        a, blockIdx.y, blockIdx.z, blockIdx.x, gridDim.z, lds_raw, s_unscale, s_w0sq, gridDim.x, s_max);
 }
 
 static int env_int(const char* name, int dflt) {
@@ -711,7 +783,8 @@ static int launch_bf(const BfArgs& args, unsigned char* ws, size_t ws_bytes, hip
     BfArgs a = args;
     a.nsteps = nsteps_for(F16X2, a.A);
     slab_sizes(F16X2, MW, a.nsteps, &a.wbytes, &a.slab_bytes, !F16X2 && MW == 2);
-    const size_t lds = a.wbytes;
+    // A operands, then the staged phasors of both pols (packed in the kernel).
+    const size_t lds = a.wbytes + (size_t)2 * MW * (TILE / 2) * a.A * sizeof(float2);
     const int span = WAVES * TILE * NT;
     const int groups = groups_for(a.B, MW);
     const size_t need = a.slab_bytes * (size_t)a.F * groups + kSinkBytes;
@@ -745,17 +818,6 @@ static int launch_bf(const BfArgs& args, unsigned char* ws, size_t ws_bytes, hip
             return BLADE_HIP_ERROR;
         }
     }
-    const size_t pack_lds = (size_t)(F16X2 ? MW : 1) * (TILE / 2) * a.A * sizeof(float2);
-    if (pack_lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&bf_pack_kernel<MW, F16X2>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)pack_lds);
-        if (e != hipSuccess) {
-            set_error(std::string("beamformer: pack LDS attribute: ") + hipGetErrorString(e));
-            return BLADE_HIP_ERROR;
-        }
-    }
-    hipLaunchKernelGGL((bf_pack_kernel<MW, F16X2>), dim3(2 * MW, (unsigned)a.F, (unsigned)groups), dim3(256),
-                       pack_lds, s, a);
     hipLaunchKernelGGL((beamform_kernel<NT, MW, CI8, F16X2, D, NTS, EPI>), grid, dim3(64 * WAVES), lds, s, a);
     return check_launch("beamformer");
 }
