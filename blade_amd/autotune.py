@@ -12,8 +12,9 @@ Candidates span the knobs of the launch:
   the three-product form);
 * ``chunks_per_item``: 256-sample time chunks per work item (load balance
   against per-item phasor staging);
-* ``workgroups_per_cu``: resident workgroups per compute unit (grid size of
-  the persistent launch).
+* ``workgroups_per_cu``: workgroups per compute unit in the grid (2-4 keep
+  every workgroup resident and walking items; 8-64 give short-lived
+  workgroups the hardware refills as they finish, up to one per item).
 
 The cache lives in ``$BLADE_AMD_CACHE`` (a directory) or
 ``~/.cache/blade_amd``; an unwritable location only disables the cache.
@@ -35,7 +36,7 @@ __all__ = ["candidates", "signature", "tune_batched", "cached_tuning"]
 
 
 def candidates():
-    for bt, cpi, wpc in itertools.product((0, 1, 2), (0, 1, 2, 4, 8), (2, 3, 4)):
+    for bt, cpi, wpc in itertools.product((0, 1, 2), (0, 1, 2, 4, 8), (2, 3, 4, 8, 16, 64)):
         yield {"time_samples_per_lane": 4, "beam_tiles": bt, "chunks_per_item": cpi,
                "workgroups_per_cu": wpc}
 

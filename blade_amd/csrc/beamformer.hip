@@ -787,16 +787,20 @@ static int launch_bf(const BfArgs& args, unsigned char* ws, size_t ws_bytes, hip
     const size_t lds = a.wbytes + (size_t)2 * MW * (TILE / 2) * a.A * sizeof(float2);
     const int span = WAVES * TILE * NT;
     const int groups = groups_for(a.B, MW);
-    const size_t need = a.slab_bytes * (size_t)a.F * groups + kSinkBytes;
+    // Device scratch: only the store sink of idle lanes (the phasors are
+    // packed into LDS by the kernel).
+    const size_t need = kSinkBytes;
+    unsigned char* scratch = nullptr;
     if (ws) {
         if (ws_bytes < need) return fail("beamformer: workspace too small");
         if ((uintptr_t)ws % 256) return fail("beamformer: workspace must be 256-byte aligned");
-        a.pack = ws;
+        scratch = ws;
     } else {
-        const int rc = library_workspace(WS_BEAMFORMER_PACK, need, s, &a.pack);
+        const int rc = library_workspace(WS_BEAMFORMER_SINK, need, s, &scratch);
         if (rc) return rc;
     }
-    a.sink = reinterpret_cast<float4*>(a.pack + (need - kSinkBytes));
+    a.pack = nullptr;
+    a.sink = reinterpret_cast<float4*>(scratch);
     a.n_tchunks = (a.T + span - 1) / span;
     // Chunks per workgroup: about 768 workgroups (one round at 3 per CU);
     // measured best for both variants at the production shape.
@@ -1138,11 +1142,8 @@ extern "C" int blade_beamform_chain(int in_type, int precision, const void* inpu
 
 extern "C" uint64_t blade_beamformer_workspace_size(int precision, uint64_t A, uint64_t F,
                                                     uint64_t B) {
-    const bool f16 = precision == BLADE_PREC_F16X2;
-    const int mw = B > 8 ? 2 : 1;
-    size_t wbytes = 0, slab = 0;
-    slab_sizes(f16, mw, nsteps_for(f16, (int)A), &wbytes, &slab, !f16 && mw == 2);
-    return (uint64_t)slab * F * groups_for((int)B, mw) + kSinkBytes;
+    (void)precision; (void)A; (void)F; (void)B;   // the same for every shape
+    return kSinkBytes;
 }
 
 extern "C" int blade_beamform_chain_ws(int in_type, int precision, const void* input,

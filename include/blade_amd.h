@@ -87,14 +87,14 @@ int blade_beamform_chain(int in_type, int precision, const void* input, const vo
                          int polarize, blade_stream_t stream);
 
 /*
- * The two calls above re-pack the phasors into the matrix-core operand
- * layout on every call, into a library-owned buffer per (device, stream).
- * Callers that own their memory (modules, hipGraph capture) size a device
- * workspace with blade_beamformer_workspace_size and pass it to
- * blade_beamform_chain_ws (same semantics as blade_beamform_chain).
- * The buffer is what the reference keeps in managed phasor memory per
- * Phasor module (src/modules/phasor/ata.cc:96); no reference counterpart
- * exists for the packed layout itself.
+ * The two calls above pack the phasors into the matrix-core operand layout
+ * inside the kernel, in LDS, on every call; their only device scratch (a
+ * 1 KiB sink for the stores of idle lanes) is a library-owned buffer per
+ * (device, stream).  Callers that own their memory (modules, hipGraph
+ * capture) size a device workspace with blade_beamformer_workspace_size
+ * and pass it to blade_beamform_chain_ws (same semantics as
+ * blade_beamform_chain).  The reference's beamformer needs no scratch
+ * (src/modules/beamformer/generic.cc:31-33); this pair has no counterpart.
  */
 uint64_t blade_beamformer_workspace_size(int precision, uint64_t A, uint64_t F, uint64_t B);
 int blade_beamform_chain_ws(int in_type, int precision, const void* input, const void* phasors,
@@ -126,7 +126,8 @@ typedef struct blade_beamform_block {
  *   beam_tiles: 16-row beam tiles per wavefront, 1 (8 beams) or 2 (16;
  *               FP32 then forms three real products per complex one)
  *   chunks_per_item: time chunks of 256 samples per work item
- *   workgroups_per_cu: resident workgroups per compute unit (grid size)
+ *   workgroups_per_cu: workgroups per compute unit in the grid (grid =
+ *               min(work items, this x CUs); items beyond it are walked)
  */
 typedef struct blade_batch_tuning {
     int32_t time_samples_per_lane;
