@@ -22,8 +22,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <atomic>
 #include <map>
 #include <stdexcept>
+#include <thread>
 #include <vector>
 
 #include "blade/logger.hh"
@@ -47,6 +49,51 @@ bool read_full(int fd, void* dst, size_t bytes, off_t offset) {
         bytes -= (size_t)r;
     }
     return true;
+}
+
+// Host threads for the reads of one step: a single thread copies from the
+// page cache at ~20 GB/s, several together at several times that.
+// BLADE_GUPPI_READ_THREADS overrides the default of 8 (1 = serial).
+unsigned read_threads() {
+    const char* v = getenv("BLADE_GUPPI_READ_THREADS");
+    const int n = v ? atoi(v) : 8;
+    return (unsigned)std::max(1, std::min(n, 64));
+}
+
+// Runs fn(i) for i in [0, n) on up to `threads` threads; true if every call
+// returned true.
+template <typename Fn>
+bool parallel_for(size_t n, unsigned threads, Fn fn) {
+    threads = (unsigned)std::min<size_t>(threads, n);
+    if (threads <= 1) {
+        for (size_t i = 0; i < n; i++)
+            if (!fn(i)) return false;
+        return true;
+    }
+    std::atomic<size_t> next{0};
+    std::atomic<bool> ok{true};
+    auto worker = [&]() {
+        for (size_t i = next++; i < n && ok.load(std::memory_order_relaxed); i = next++)
+            if (!fn(i)) ok = false;
+    };
+    std::vector<std::thread> pool;
+    pool.reserve(threads - 1);
+    for (unsigned k = 1; k < threads; k++) pool.emplace_back(worker);
+    worker();
+    for (auto& t : pool) t.join();
+    return ok.load();
+}
+
+// read_full over `threads` threads in pieces of at least 4 MiB.
+bool read_full_parallel(int fd, void* dst, size_t bytes, off_t offset, unsigned threads) {
+    constexpr size_t kPiece = 4u << 20;
+    const size_t n = std::max<size_t>(1, std::min<size_t>(threads * 4, bytes / kPiece));
+    const size_t step = (bytes + n - 1) / n;
+    unsigned char* p = static_cast<unsigned char*>(dst);
+    return parallel_for(n, threads, [&](size_t i) {
+        const size_t b = i * step, e = std::min(bytes, b + step);
+        return b >= e || read_full(fd, p + b, e - b, offset + (off_t)b);
+    });
 }
 
 bool write_full(int fd, const void* src, size_t bytes) {
@@ -300,25 +347,29 @@ Result Reader<OT>::process(const U64&, const Stream&) {
     const size_t sample = st.n_pol * 2;                      // bytes per time sample
     unsigned char* dst = reinterpret_cast<unsigned char*>(output.stepBuffer.data());
     const bool whole = nA == st.n_aspect && nF == st.n_aspectchan && nT == st.n_time && st.t0 % st.n_time == 0;
+    // Rows (and pieces of a whole block) are read by several host threads.
+    const unsigned threads = read_threads();
     if (whole) {
-        if (!read_full(st.fds[bm.file], dst, nA * nF * nT * sample, bm.data_offset)) return Result::ERROR;
+        if (!read_full_parallel(st.fds[bm.file], dst, nA * nF * nT * sample, bm.data_offset, threads))
+            return Result::ERROR;
     } else {
-        for (U64 a = 0; a < nA; a++) {
-            for (U64 f = 0; f < nF; f++) {
-                U64 t = st.t0, done = 0;
-                while (done < nT) {
-                    const U64 b = t / st.n_time, tb = t % st.n_time;
-                    const U64 run = std::min(nT - done, st.n_time - tb);
-                    const Block& blk = st.blocks[b];
-                    const off_t off = blk.data_offset +
-                        (off_t)((((st.a0 + a) * st.n_aspectchan + (st.f0 + f)) * st.n_time + tb) * sample);
-                    if (!read_full(st.fds[blk.file], dst + ((a * nF + f) * nT + done) * sample, run * sample, off))
-                        return Result::ERROR;
-                    done += run;
-                    t += run;
-                }
+        const bool ok = parallel_for(nA * nF, threads, [&](size_t row) {
+            const U64 a = row / nF, f = row % nF;
+            U64 t = st.t0, done = 0;
+            while (done < nT) {
+                const U64 b = t / st.n_time, tb = t % st.n_time;
+                const U64 run = std::min(nT - done, st.n_time - tb);
+                const Block& blk = st.blocks[b];
+                const off_t off = blk.data_offset +
+                    (off_t)((((st.a0 + a) * st.n_aspectchan + (st.f0 + f)) * st.n_time + tb) * sample);
+                if (!read_full(st.fds[blk.file], dst + ((a * nF + f) * nT + done) * sample, run * sample, off))
+                    return false;
+                done += run;
+                t += run;
             }
-        }
+            return true;
+        });
+        if (!ok) return Result::ERROR;
     }
 
     st.t0 += nT;
