@@ -410,6 +410,28 @@ def test_permutation_bit_exact(idx, dtype):
     assert np.array_equal(y, ref)
 
 
+@pytest.mark.parametrize("shape,idx,dtype", [
+    ((2, 192, 8192, 2), [0, 2, 3, 1], "cf32"),     # AFTP -> ATPF at the beams' shape
+    ((2, 100, 130, 2), [0, 2, 3, 1], "cf32"),      # ragged 64-element tiles
+    ((2, 100, 130, 2), [3, 2, 1, 0], "cf32"),
+    ((3, 70, 97, 1), [2, 0, 3, 1], "cf32"),
+    ((1, 70, 300, 2), [0, 2, 3, 1], "ci8"),        # 2-byte elements
+    ((2, 65, 66, 2), [1, 3, 2, 0], "cf16")])
+def test_permutation_wide_tiles_bit_exact(shape, idx, dtype):
+    rng = np.random.default_rng(len(shape) + sum(idx) + shape[1])
+    if dtype == "cf32":
+        x = crand(rng, *shape)
+        ref = O.permutation(x, idx)
+    elif dtype == "ci8":
+        x = rng.integers(-128, 128, size=shape + (2,)).astype(np.int8)
+        ref = np.ascontiguousarray(np.transpose(x, idx + [4]))
+    else:
+        x = rng.normal(size=shape + (2,)).astype(np.float16)
+        ref = np.ascontiguousarray(np.transpose(x, idx + [4]))
+    y = host(ops.permutation(dev(x), idx, type_name=dtype))
+    assert np.array_equal(y, ref)
+
+
 # ---------------------------------------------------------------------- phasor
 
 @pytest.mark.parametrize("B,A,F,tau_max,f_obs", [(2, 20, 192, 5e-6, 6500.125e6),
