@@ -275,7 +275,9 @@ static int launch_permutation(const void* in, void* out, const Perm4& p, hipStre
     int r = -1;
     for (int d = 0; d < 3; d++)
         if (p.in_stride_of_out_dim[d] == 1 && p.out_shape[d] > 1) r = d;
-    if (r >= 0 && p.in_stride_of_out_dim[3] != 1 && p.out_shape[3] > 1) {
+    // The tiled transpose when both tile axes fill most of a 32 x 32 tile;
+    // otherwise the direct gather (coalesced stores, loads through L2).
+    if (r >= 0 && p.in_stride_of_out_dim[3] != 1 && p.out_shape[3] >= 16 && p.out_shape[r] >= 16) {
         uint64_t outer = 1;
         for (int d = 0; d < 3; d++)
             if (d != r) outer *= p.out_shape[d];
