@@ -17,6 +17,10 @@ The ``cpu_baseline`` leg times the vectorised numpy port of the same chain
 on the host's cores and checks both GPU variants against the complex128
 oracle on a few channels.
 
+``unfused_f32`` times the same block as one plain FP32 kernel per reference
+module (cast, beamformer, channelizer, polarizer); ``split_f16`` reports its
+speedup over both that chain and the fused FP32 kernel.
+
 Extra legs (not the headline value; ``--no-extra`` skips them):
 ``batched`` -- 32 uneven blocks in one persistent launch, autotuned;
 ``mode_a`` -- split-FP16 beamform -> FFT-8192 -> Stokes I on A28 F192 T16384 B8;
@@ -181,6 +185,36 @@ def _ci8(shape, gen):
     import torch
     v = torch.randn(tuple(shape) + (2,), device="cuda", generator=gen) * 16.0
     return torch.clamp(torch.round(v), -127, 127).to(torch.int8)
+
+
+def unfused_leg(iters=20):
+    """The same Mode-B block through one plain FP32 kernel per reference
+    module, in the reference's order (Cast CI8 -> CF32, Beamformer, Channelizer
+    at rate 4, Polarizer in place), with this library's kernels and the
+    voltage blocks cycled as in the main leg.  Device time per block."""
+    import torch
+    from blade_amd import ops
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    xs = [_ci8((A, F, T, P), gen) for _ in range(NSETS)]
+    w = torch.randn((B, A, F, 1, P), dtype=torch.complex64, device="cuda", generator=gen)
+    xf = torch.empty((A, F, T, P), dtype=torch.complex64, device="cuda")
+    y = torch.empty((B, F, T, P), dtype=torch.complex64, device="cuda")
+    ch = torch.empty((B, F * RATE, T // RATE, P), dtype=torch.complex64, device="cuda")
+    k = [0]
+
+    def chain():
+        x = xs[k[0] % NSETS]
+        k[0] += 1
+        ops.cast(x, "cf32", out=xf, in_type="ci8")
+        ops.beamformer(xf, w, False, False, "f32", out=y)
+        ops.channelizer(y, RATE, out=ch)
+        ops.polarizer(ch, out=ch)
+
+    _events_ms(chain, 3)
+    ms = _events_ms(chain, iters)
+    return {"workload": "Mode-B block as four FP32 kernels: cast CI8->CF32, beamform (CF32 in), "
+                        "channelize rate 4, polarize in place",
+            "ms_per_block": round(ms, 4)}
 
 
 def batched_leg(iters=20):
@@ -475,6 +509,8 @@ def main():
         "speedup_vs_f32": round(f32["kernel_ms"] / h["kernel_ms"], 3),
         "rel_err_vs_f32": rel_f16_vs_f32,
     }
+    unfused = unfused_leg()
+    split["speedup_vs_unfused_f32"] = round(unfused["ms_per_block"] / h["kernel_ms"], 3)
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "Msamples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(f32["step_ms"], 5),
@@ -486,6 +522,7 @@ def main():
                    "input": "CI8 N(0,16^2) rounded+clipped", "parallelism": "single GPU"},
         "roofline": roof,
         "split_f16": split,
+        "unfused_f32": unfused,
         "realtime_factor": round((T / 0.5e6) / t_step, 1),
     }
     if not args.no_extra:
