@@ -28,7 +28,9 @@ Extra legs (not the headline value; ``--no-extra`` skips them):
 ``kernels`` -- achieved HBM rate of the bandwidth-bound kernels (polarizer,
 detector, cast, gather, permutation, channelizer) on the block's beams;
 ``ingest`` -- GUPPI RAW blocks read into pinned memory, copied to HBM and
-beamformed, per stage and end to end.
+beamformed, per stage and end to end;
+``reference_benchmarks`` -- every module at the arguments of the reference's
+own benchmark suite (benchmarks/modules).
 """
 
 from __future__ import annotations
@@ -345,6 +347,87 @@ def kernels_leg(iters=20):
     return out
 
 
+def reference_benchmarks_leg(iters=20):
+    """Each module at the arguments of the reference's own benchmark suite
+    (benchmarks/modules/*/generic.hh, beamformer/ata.hh): the same shapes,
+    one process() per iteration on resident buffers, as the reference times
+    them.  Inputs under 256 MB stay in the Infinity Cache across iterations,
+    as they stay in a GPU's L2 in the reference's runs.  Algorithmic bytes =
+    read + written once."""
+    import torch
+    from blade_amd import ops
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    out = {}
+
+    def cf32(*shape):
+        return torch.randn(shape, dtype=torch.complex64, device="cuda", generator=gen)
+
+    def rate(name, fn, nbytes):
+        _events_ms(fn, 3)
+        ms = _events_ms(fn, iters)
+        out[name] = {"us": round(ms * 1e3, 1), "gbs": round(nbytes / ms / 1e6, 1)}
+
+    for b_, a_ in ((1, 20), (2, 20), (8, 20), (1, 42)):      # beamformer/ata.hh:15-18
+        x = cf32(a_, 192, 8192, 2)
+        w = cf32(b_, a_, 192, 1, 2)
+        y = torch.empty((b_, 192, 8192, 2), dtype=torch.complex64, device="cuda")
+        rate(f"beamformer_cf32_B{b_}_A{a_}", lambda: ops.beamformer(x, w, False, False, "f32", out=y),
+             x.numel() * 8 + w.numel() * 8 + y.numel() * 8)
+        del x, w, y
+    for a_ in (2, 16, 64):                                       # cast/generic.hh:15-49
+        n = a_ * 192 * 8192 * 2
+        x = cf32(a_, 192, 8192, 2)
+        y = torch.empty_like(x)
+        rate(f"cast_cf32_cf32_A{a_}", lambda: ops.cast(x, "cf32", out=y), 16 * n)
+        del x
+        x8 = _ci8((a_, 192, 8192, 2), gen)
+        rate(f"cast_ci8_cf32_A{a_}", lambda: ops.cast(x8, "cf32", out=y, in_type="ci8"), 10 * n)
+        del y
+        h = ops.cast(x8, "cf16", in_type="ci8")
+        rate(f"cast_ci8_cf16_A{a_}", lambda: ops.cast(x8, "cf16", out=h, in_type="ci8"), 6 * n)
+        del x8, h
+    x = cf32(16, 192, 8192, 2)                                   # channelizer/generic.hh:15
+    y = torch.empty_like(x)
+    rate("channelizer_A16_C8192", lambda: ops.channelizer(x, 8192, out=y), 2 * x.numel() * 8)
+    del y
+    for a_, i_, p_ in ((2, 8, 4), (16, 8, 4), (2, 32, 4), (16, 32, 4), (2, 64, 4), (16, 64, 4),
+                       (2, 64, 1), (16, 64, 1)):                 # detector/generic.hh:15-37
+        xa = x[:a_]
+        d = torch.empty((a_, 192, 8192 // i_, p_), dtype=torch.float32, device="cuda")
+        rate(f"detector_A{a_}_I{i_}_P{p_}", lambda: ops.detector(xa, i_, p_, out=d),
+             xa.numel() * 8 + d.numel() * 4)
+    del x
+    x = cf32(20, 192, 8192, 2)                                   # duplicate/generic.hh:13-25
+    y = torch.empty_like(x)
+    rate("duplicate_cf32", lambda: ops.gather(x, y, 0, 1, 0), 2 * x.numel() * 8)
+    xr = torch.view_as_real(x)[..., 0].contiguous()
+    yr = torch.empty_like(xr)
+    rate("duplicate_f32", lambda: ops.gather(xr, yr, 0, 1, 0), 2 * xr.numel() * 4)
+    del x, y, xr, yr
+    for f_, t_ in ((192, 8192), (131072, 1)):                    # gather/generic.hh:15-30
+        x = cf32(5, f_, t_, 2)
+        y = torch.empty((5, f_, t_ * 64, 2), dtype=torch.complex64, device="cuda")
+        rate(f"gather_cf32_x64_F{f_}_T{t_}", lambda: ops.gather(x, y, 2, 64, 3), 2 * x.numel() * 8)
+        xh = torch.randn((5, f_, t_, 2), device="cuda", generator=gen).to(torch.float16)
+        yh = torch.empty((5, f_, t_ * 64, 2), dtype=torch.float16, device="cuda")
+        rate(f"gather_f16_x64_F{f_}_T{t_}", lambda: ops.gather(xh, yh, 2, 64, 3, type_name="f16"),
+             2 * xh.numel() * 2)
+        del x, y, xh, yh
+    x = cf32(2, 192, 8192, 2)                                    # permutation/generic.hh:13-15
+    for idx in ((0, 1, 2, 3), (0, 2, 1, 3), (0, 2, 3, 1)):
+        rate("permutation_" + "".join("AFTP"[i] for i in idx), lambda: ops.permutation(x, idx),
+             2 * x.numel() * 8)
+    for a_ in (2, 16):                                           # polarizer/generic.hh:15-30
+        xa = cf32(a_, 192, 8192, 2)
+        ya = torch.empty_like(xa)
+        rate(f"polarizer_cf32_A{a_}", lambda: ops.polarizer(xa, out=ya), 2 * xa.numel() * 8)
+        xh = ops.cast(xa, "cf16")
+        yh = torch.empty_like(xh)
+        rate(f"polarizer_cf16_A{a_}", lambda: ops.polarizer(xh, out=yh), 2 * xa.numel() * 4)
+        del xa, ya, xh, yh
+    return out
+
+
 def ingest_leg(nblocks=6):
     """Ingest of GUPPI RAW blocks of the Mode-B shape (A28 F192 T8192, CI8,
     176 MB per block) from a file in the page cache: the reader fills its
@@ -531,6 +614,7 @@ def main():
         line["calibration"] = _extra(calibration_leg)
         line["kernels"] = _extra(kernels_leg)
         line["ingest"] = _extra(ingest_leg)
+        line["reference_benchmarks"] = _extra(reference_benchmarks_leg)
     if not args.no_cpu:
         x, w, outs = bufs
         cpu, errs = cpu_leg(x, w, outs)

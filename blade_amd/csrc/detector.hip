@@ -90,7 +90,10 @@ static unsigned grid_cap(uint64_t blocks) {
 template <int NPOL>
 static void launch(const float4* in, float* out, uint64_t n_out, uint64_t intg, int acc,
                    hipStream_t s) {
-    if (intg >= 64) {
+    // 64 lanes per output only where outputs are too few to fill the chip
+    // with 16: a 64-lane reduction costs four times the shuffles per byte
+    // (at INTG 64 with 4 pols, 3.7 against 5.7 TB/s).
+    if (intg >= 64 && n_out < 65536) {
         hipLaunchKernelGGL((detector_group_kernel<NPOL, 64>), dim3(grid_cap(ceil_div(n_out, 4))),
                            dim3(256), 0, s, in, out, n_out, intg, acc);
     } else if (intg >= 16) {

