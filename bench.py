@@ -30,7 +30,9 @@ detector, cast, gather, permutation, channelizer) on the block's beams;
 ``ingest`` -- GUPPI RAW blocks read into pinned memory, copied to HBM and
 beamformed, per stage and end to end;
 ``reference_benchmarks`` -- every module at the arguments of the reference's
-own benchmark suite (benchmarks/modules).
+own benchmark suite (benchmarks/modules);
+``mode_b_bundle`` -- the reference's Mode-B bundle benchmark configuration
+through a Runner with host buffers, per block.
 """
 
 from __future__ import annotations
@@ -428,6 +430,101 @@ def reference_benchmarks_leg(iters=20):
     return out
 
 
+# ITRF positions (m) of the 20 ATA antennas of the reference's Mode-B
+# benchmark (benchmarks/bundles/ata/mode-b/mode_b.hh:70-89).
+ATA20 = [(-2524041.5388905862, -4123587.965024342, 4147646.4222955606),
+         (-2524068.187873109, -4123558.735413135, 4147656.21282186),
+         (-2524087.2078100787, -4123532.397416349, 4147670.9866770394),
+         (-2524103.384010733, -4123511.111598937, 4147682.4133068994),
+         (-2524056.730228759, -4123515.287949227, 4147706.4850287656),
+         (-2523986.279601761, -4123497.427940991, 4147766.732988923),
+         (-2523970.301363642, -4123515.238502669, 4147758.790023165),
+         (-2523983.5419911123, -4123528.1422073604, 4147737.872218138),
+         (-2523941.5221860334, -4123568.125040547, 4147723.8292249846),
+         (-2524074.096220788, -4123468.5182652213, 4147742.0422435375),
+         (-2524058.6409591637, -4123466.5112451194, 4147753.4513993543),
+         (-2524026.989692545, -4123480.9405167866, 4147758.2356800516),
+         (-2524048.5254066754, -4123468.3463909747, 4147757.835369889),
+         (-2524000.5641107005, -4123498.2984570004, 4147756.815976133),
+         (-2523945.086670364, -4123480.3638816103, 4147808.127865142),
+         (-2523950.6822576034, -4123444.7023326857, 4147839.7474427638),
+         (-2523880.869769226, -4123514.3375464156, 4147813.413426994),
+         (-2523930.3747946257, -4123454.3080821196, 4147842.6449955846),
+         (-2523898.1150373477, -4123456.314794732, 4147860.3045849088),
+         (-2523824.598229116, -4123527.93080514, 4147833.98936114)]
+
+
+def mode_b_bundle_leg(blocks=10):
+    """The reference's Mode-B bundle benchmark configuration
+    (benchmarks/bundles/ata/mode-b/mode_b.hh:55-115): CI8 [20,192,8192,2] ->
+    delays and phasors for 8 beams + incoherent beam -> beamform -> Stokes I
+    -> F32 [9,192,8192,1], through a Runner with host buffers (copy in, the
+    bundle, copy out, synchronously per block).  Wall time per block."""
+    import math
+    import blade_amd as bl
+    in_shape, out_shape = (20, 192, 8192, 2), (9, 192, 8192, 1)
+    cal = bl.array_tensor((20, 192, 1, 2), dtype=bl.cf64, device=bl.cpu)
+    np.copyto(cal.as_numpy(), np.ones((20, 192, 1, 2), dtype=np.complex128))
+    config = {
+        "input_shape": in_shape, "output_shape": out_shape,
+        "pre_beamformer_channelizer_rate": 1,
+        "phasor_observation_frequency_hz": 6500.125e6, "phasor_channel_bandwidth_hz": 0.5e6,
+        "phasor_total_bandwidth_hz": 1.024e9, "phasor_frequency_start_index": 352,
+        "phasor_reference_antenna_index": 0,
+        "phasor_array_reference_position": bl.lla(lon=math.radians(-121.470733),
+                                                  lat=math.radians(40.815987), alt=1020.86),
+        "phasor_boresight_coordinate": bl.ra_dec(ra=0.64169, dec=1.079896295),
+        "phasor_antenna_positions": [bl.xyz(*q) for q in ATA20],
+        "phasor_antenna_calibrations": cal,
+        "phasor_beam_coordinates": [bl.ra_dec(ra=0.63722, dec=1.07552424)] +
+                                   [bl.ra_dec(ra=0.64169, dec=1.079896295)] * 7,
+        "beamformer_incoherent_beam": True,
+        "detector_enable": True, "detector_integration_size": 1,
+        "detector_number_of_output_polarizations": 1,
+    }
+
+    @bl.runner
+    class ModeB:
+        def __init__(self):
+            self.input.dut = bl.tensor(1, dtype=bl.f64, device=bl.cpu)
+            self.input.date = bl.tensor(1, dtype=bl.f64, device=bl.cpu)
+            self.input.buffer = bl.array_tensor(in_shape, dtype=bl.ci8)
+            self.output.buffer = bl.array_tensor(out_shape, dtype=bl.f32)
+            self.module.mode_b = bl.module(bl.modeb, config,
+                                           (self.input.dut, self.input.date, self.input.buffer),
+                                           it=bl.ci8, ot=bl.f32, telescope=bl.ata)
+
+        def transfer_in(self, dut, date, buffer):
+            self.copy(self.input.dut, dut)
+            self.copy(self.input.date, date)
+            self.copy(self.input.buffer, buffer)
+
+        def transfer_out(self, buffer):
+            self.copy(self.output.buffer, self.module.mode_b.get_output())
+            self.copy(buffer, self.output.buffer)
+
+    dut = bl.tensor(1, dtype=bl.f64, device=bl.cpu)
+    date = bl.tensor(1, dtype=bl.f64, device=bl.cpu)
+    host_in = bl.array_tensor(in_shape, dtype=bl.ci8, device=bl.cpu)
+    host_out = bl.array_tensor(out_shape, dtype=bl.f32, device=bl.cpu)
+    rng = np.random.default_rng(3)
+    np.copyto(host_in.as_numpy(), np.clip(np.round(rng.normal(size=in_shape + (2,)) * 16), -127, 127)
+              .astype(np.int8))
+    dut.as_numpy()[0] = 0.0
+    p = ModeB()
+    times = []
+    for k in range(blocks + 2):
+        date.as_numpy()[0] = (1649366473.0 + k * 0.016384) / 86400 + 2440587.5
+        t0 = time.perf_counter()
+        p(dut, date, host_in, host_out)
+        times.append(time.perf_counter() - t0)
+    ms = float(np.median(times[2:])) * 1e3
+    return {"workload": "reference Mode-B benchmark: CI8 [20,192,8192,2] -> 8 beams + incoherent, "
+                        "Stokes I -> F32 [9,192,8192,1], host buffers in and out",
+            "ms_per_block": round(ms, 3), "realtime_factor": round(16.384 / ms, 2),
+            "finite": bool(np.isfinite(host_out.as_numpy()).all())}
+
+
 def ingest_leg(nblocks=6):
     """Ingest of GUPPI RAW blocks of the Mode-B shape (A28 F192 T8192, CI8,
     176 MB per block) from a file in the page cache: the reader fills its
@@ -615,6 +712,7 @@ def main():
         line["kernels"] = _extra(kernels_leg)
         line["ingest"] = _extra(ingest_leg)
         line["reference_benchmarks"] = _extra(reference_benchmarks_leg)
+        line["mode_b_bundle"] = _extra(mode_b_bundle_leg)
     if not args.no_cpu:
         x, w, outs = bufs
         cpu, errs = cpu_leg(x, w, outs)
