@@ -157,6 +157,20 @@ def cpu_leg(x, w, outs, budget_s=12.0):
             break
     t_med = float(np.median(times))
     value = B * fs * T * P / t_med / 1e6
+    # The same sample on one thread (a shorter run: median of >= 3).
+    fs1 = 4
+    x1, w1 = x_np[:, :fs1], w_np[:, :, :fs1]
+    with threadpoolctl.threadpool_limits(limits=1):
+        O.beamform_chain_c64(x1, w1, RATE, True)
+        t1s = []
+        t0 = time.perf_counter()
+        while (time.perf_counter() - t0 < budget_s / 2) or len(t1s) < 3:
+            t1 = time.perf_counter()
+            O.beamform_chain_c64(x1, w1, RATE, True)
+            t1s.append(time.perf_counter() - t1)
+            if len(t1s) >= 30:
+                break
+    value_1t = B * fs1 * T * P / float(np.median(t1s)) / 1e6
     # parity on 2 channels against the complex128 oracle
     fc = 2
     ref = O.polarizer(O.channelizer(O.beamformer_ata(x[:, :fc].cpu().numpy(), w[:, :, :fc].cpu().numpy()), RATE))
@@ -170,6 +184,13 @@ def cpu_leg(x, w, outs, budget_s=12.0):
                   f"A{A} F{F} T{T} B{B} block, median of {len(times)} runs "
                   f"({sum(times):.1f} s), {threads} BLAS threads; scaled per beamformed sample",
         "cpu_model": _cpu_model(),
+        "single_thread": {"value": round(value_1t, 2), "unit": "Msamples/s", "cores": 1,
+                          "sample": f"F={fs1}, median of {len(t1s)} runs ({sum(t1s):.1f} s)"},
+        "note": "numpy's elementwise complex operations run on one thread; only BLAS calls "
+                "use the others, so the two runs are expected to be close",
+        "host": {"os_cpu_count": os.cpu_count(), "affinity_cores": len(os.sched_getaffinity(0)),
+                 "blas_env": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS",
+                                                              "MKL_NUM_THREADS")}},
     }, errs
 
 
