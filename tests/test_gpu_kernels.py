@@ -51,6 +51,9 @@ BF_CASES = [
     (27, 2, 1024, 33, True, False),    # three beam tiles
     (14, 2, 250, 3, False, False),     # T % 4 != 0
     (64, 2, 256, 64, True, True),
+    (128, 2, 512, 16, True, True),     # the incoherent beam's antenna limit
+    (160, 1, 256, 16, False, False),   # large phasor stages in LDS
+    (200, 1, 256, 8, False, False),
 ]
 
 
@@ -133,6 +136,14 @@ def test_beamform_chain_caller_workspace(precision):
             _lib.call("blade_beamform_chain_ws", _lib.CI8, prec, ct.c_void_p(x8.data_ptr()),
                       ct.c_void_p(w.data_ptr()), ct.c_void_p(out.data_ptr()), A, F, T, 2, B, 1, 0, 4, 1,
                       ct.c_void_p(ws.data_ptr()), nbytes - 1, ops.stream_handle())
+
+
+def test_beamformer_rejects_too_many_antennas_for_lds():
+    # 16 beams x 400 antennas of packed phasors exceed the LDS stage.
+    x = torch.zeros((400, 1, 256, 2, 2), dtype=torch.int8, device="cuda")
+    w = torch.zeros((16, 400, 1, 1, 2), dtype=torch.complex64, device="cuda")
+    with pytest.raises(BladeError, match="too many antennas"):
+        ops.beamformer(x, w, False, False, "f32")
 
 
 def test_beamformer_rejects_bad_shapes():
