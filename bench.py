@@ -47,6 +47,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The modules' INFO lines go to stdout; the bench's stdout is one JSON line.
+os.environ.setdefault("BLADE_QUIET", "1")
 
 METRIC = "beamformed Msamples/s + % HBM/MFMA roofline; split-FP16 vs FP32 @ ≤1e-5 err"
 A, F, T, P, B, RATE = 28, 192, 8192, 2, 16, 4
