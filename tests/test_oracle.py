@@ -348,3 +348,28 @@ def test_cpu_port_matches_oracle():
         got = O.beamform_chain_c64(x8, w, rate, pol)
         assert got.dtype == np.complex64
         assert O.norm_rel_err(got, ref) < 1e-6
+
+
+def test_mode_h_closed_forms():
+    """Mode H (channelizer at rate T, polarizer, detector), pinned without
+    restating its composition: a tone lands in its fftshifted bin with power
+    C^2 (|X|^2 + |Y|^2); Parseval holds over the channels; circular
+    conversion doubles Stokes I (|X + jY|^2 + |X - jY|^2 = 2 (|X|^2 + |Y|^2))."""
+    C, k = 64, 5
+    t = np.arange(C)
+    ax, ay = 0.75 - 0.5j, -0.25 + 1.0j
+    x = np.zeros((1, 1, C, 2), dtype=np.complex128)
+    x[0, 0, :, 0] = ax * np.exp(2j * np.pi * k * t / C)
+    x[0, 0, :, 1] = ay * np.exp(2j * np.pi * k * t / C)
+    y = O.mode_h(x)                                         # [1, C, 1, 1] Stokes I
+    assert y.shape == (1, C, 1, 1)
+    expect = np.zeros(C)
+    expect[k + C // 2] = C * C * (abs(ax) ** 2 + abs(ay) ** 2)
+    np.testing.assert_allclose(y[0, :, 0, 0], expect, rtol=1e-12, atol=1e-9)
+
+    rng = np.random.default_rng(4)
+    z = rng.normal(size=(2, 3, 32, 2)) + 1j * rng.normal(size=(2, 3, 32, 2))
+    s = O.mode_h(z)                                         # [2, 3 * 32, 1, 1]
+    per = s.reshape(2, 3, 32).sum(axis=2)
+    np.testing.assert_allclose(per, 32 * (np.abs(z) ** 2).sum(axis=(2, 3)), rtol=1e-12)
+    np.testing.assert_allclose(O.mode_h(z, convert_to_circular=True), 2 * s, rtol=1e-12)
