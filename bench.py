@@ -31,8 +31,9 @@ detector, cast, gather, permutation, channelizer) on the block's beams;
 beamformed, per stage and end to end;
 ``reference_benchmarks`` -- every module at the arguments of the reference's
 own benchmark suite (benchmarks/modules);
-``mode_b_bundle`` -- the reference's Mode-B bundle benchmark configuration
-through a Runner with host buffers, per block.
+``mode_b_bundle``, ``mode_h_bundle`` -- the reference's Mode-B and Mode-H
+bundle benchmark configurations through a Runner with host buffers, per
+block.
 """
 
 from __future__ import annotations
@@ -548,6 +549,57 @@ def mode_b_bundle_leg(blocks=10):
             "finite": bool(np.isfinite(host_out.as_numpy()).all())}
 
 
+def mode_h_bundle_leg(blocks=10):
+    """The reference's Mode-H bundle benchmark configuration
+    (benchmarks/bundles/generic/mode-h/mode_h.hh:47-55): CF32 [8,192,8192,2]
+    -> channelizer at rate 8192 -> XY -> LR -> Stokes I -> F32
+    [8,1572864,1,1], through a Runner with host buffers, one synchronous call
+    per block (wall time), and the fused device kernel alone (HIP events)."""
+    import torch
+    import blade_amd as bl
+    from blade_amd import ops
+    in_shape, out_shape = (8, 192, 8192, 2), (8, 192 * 8192, 1, 1)
+    config = {"input_shape": in_shape, "output_shape": out_shape,
+              "polarizer_convert_to_circular": True,
+              "detector_integration_size": 1, "detector_number_of_output_polarizations": 1}
+
+    @bl.runner
+    class ModeH:
+        def __init__(self):
+            self.input.buffer = bl.array_tensor(in_shape, dtype=bl.cf32)
+            self.output.buffer = bl.array_tensor(out_shape, dtype=bl.f32)
+            self.module.mode_h = bl.module(bl.modeh, config, self.input.buffer, it=bl.cf32, ot=bl.f32)
+
+        def transfer_in(self, buffer):
+            self.copy(self.input.buffer, buffer)
+
+        def transfer_out(self, buffer):
+            self.copy(self.output.buffer, self.module.mode_h.get_output())
+            self.copy(buffer, self.output.buffer)
+
+    host_in = bl.array_tensor(in_shape, dtype=bl.cf32, device=bl.cpu)
+    host_out = bl.array_tensor(out_shape, dtype=bl.f32, device=bl.cpu)
+    rng = np.random.default_rng(8)
+    np.copyto(host_in.as_numpy(), (rng.random(in_shape) + 1j * rng.random(in_shape)).astype(np.complex64))
+    p = ModeH()
+    times = []
+    for _ in range(blocks + 2):
+        t0 = time.perf_counter()
+        p(host_in, host_out)
+        times.append(time.perf_counter() - t0)
+    ms = float(np.median(times[2:])) * 1e3
+    x = torch.randn(in_shape, dtype=torch.complex64, device="cuda")
+    d = torch.empty((8, 192 * 8192, 1, 1), dtype=torch.float32, device="cuda")
+    fn = lambda: ops.channelizer_detector(x, 8192, 1, 1, out=d)
+    _events_ms(fn, 3)
+    dev_ms = _events_ms(fn, 20)
+    return {"workload": "reference Mode-H benchmark: CF32 [8,192,8192,2] -> FFT 8192 -> XY->LR -> "
+                        "Stokes I -> F32 [8,1572864,1,1], host buffers in and out",
+            "ms_per_block": round(ms, 3), "realtime_factor": round(16.384 / ms, 2),
+            "fused_kernel_us": round(dev_ms * 1e3, 1),
+            "finite": bool(np.isfinite(host_out.as_numpy()).all())}
+
+
 def ingest_leg(nblocks=6):
     """Ingest of GUPPI RAW blocks of the Mode-B shape (A28 F192 T8192, CI8,
     176 MB per block) from a file in the page cache: the reader fills its
@@ -736,6 +788,7 @@ def main():
         line["ingest"] = _extra(ingest_leg)
         line["reference_benchmarks"] = _extra(reference_benchmarks_leg)
         line["mode_b_bundle"] = _extra(mode_b_bundle_leg)
+        line["mode_h_bundle"] = _extra(mode_h_bundle_leg)
     if not args.no_cpu:
         x, w, outs = bufs
         cpu, errs = cpu_leg(x, w, outs)
