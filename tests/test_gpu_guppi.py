@@ -142,9 +142,9 @@ def test_reader_rejects_bad_files(tmp_path):
         reader(tmp_path / "rec.0000.raw", 64, 3, 1)     # 3 channels do not tile 4
 
 
-@pytest.mark.parametrize("dtype,np_type", [("cf32", np.complex64)])
+@pytest.mark.parametrize("dtype,nbits", [("cf32", 32), ("cf16", 16)])
 @pytest.mark.parametrize("directio", [True, False])
-def test_writer_blocks_and_header(tmp_path, dtype, np_type, directio):
+def test_writer_blocks_and_header(tmp_path, dtype, nbits, directio):
     rng = np.random.default_rng(5)
     shape = (2, 3, 16, 2)
     buf = bl.array_tensor(shape, dtype=getattr(bl, dtype), device=bl.cpu)
@@ -154,7 +154,11 @@ def test_writer_blocks_and_header(tmp_path, dtype, np_type, directio):
     w.header_put("SCHAN", 7)
     frames = []
     for k in range(3):
-        v = (rng.normal(size=shape) + 1j * rng.normal(size=shape)).astype(np_type)
+        v = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+        if dtype == "cf32":
+            v = v.astype(np.complex64)
+        else:           # CF16 host tensors are float16 pairs (trailing axis of 2)
+            v = np.stack([v.real, v.imag], axis=-1).astype(np.float16)
         np.copyto(buf.as_numpy(), v)
         frames.append(v.copy())
         assert w.process(k) == bl.result.success
@@ -164,11 +168,12 @@ def test_writer_blocks_and_header(tmp_path, dtype, np_type, directio):
     for k, (cards, payload) in enumerate(blocks):
         assert int(cards["BLOCSIZE"]) == frames[k].nbytes
         assert int(cards["OBSNCHAN"]) == 6 and int(cards["NANTS"]) == 2 and int(cards["NBEAM"]) == 2
-        assert int(cards["NPOL"]) == 2 and int(cards["NBITS"]) == 32
+        assert int(cards["NPOL"]) == 2 and int(cards["NBITS"]) == nbits
         assert int(cards["DIRECTIO"]) == int(directio)
         assert cards["DATATYPE"] == "FLOAT"
         assert float(cards["OBSFREQ"]) == 1420.0 and int(cards["SCHAN"]) == 7
-        assert np.array_equal(np.frombuffer(payload, dtype=np_type).reshape(shape), frames[k])
+        assert np.array_equal(np.frombuffer(payload, dtype=frames[k].dtype).reshape(frames[k].shape),
+                              frames[k])
     # reference writer.cc:59: PKTIDX = samples * (blocks written before the previous one)
     assert "PKTIDX" not in blocks[0][0]
     assert [int(b[0]["PKTIDX"]) for b in blocks[1:]] == [0, 16]
