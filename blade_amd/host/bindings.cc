@@ -130,6 +130,7 @@ void bind_device(py::module_& m, const char* name) {
     bind_device_type<D, F16>(mm, "f16");
     bind_device_type<D, F32>(mm, "f32");
     bind_device_type<D, F64>(mm, "f64");
+    bind_device_type<D, I32>(mm, "i32");     // output of the F32 -> I32 cast
     bind_device_type<D, CI8>(mm, "ci8");
     bind_device_type<D, CF16>(mm, "cf16");
     bind_device_type<D, CF32>(mm, "cf32");
@@ -639,18 +640,29 @@ PYBIND11_MODULE(_blade_ext, m) {
     auto de = m.def_submodule("_detector_impl");
     bind_detector<CF32, F32>(de, "in_cf32", "out_f32");
 
+    // The reference's 22 pairs (python/blade/ext_cast.cc:43-75); the pairs
+    // between real and complex types throw at construction, as there.
     auto ca = m.def_submodule("_cast_impl");
     bind_cast<I8, I8>(ca, "in_i8", "out_i8");
     bind_cast<I8, F32>(ca, "in_i8", "out_f32");
-    bind_cast<F16, F16>(ca, "in_f16", "out_f16");
     bind_cast<F16, F32>(ca, "in_f16", "out_f32");
-    bind_cast<F32, F32>(ca, "in_f32", "out_f32");
+    bind_cast<F16, CF32>(ca, "in_f16", "out_cf32");
+    bind_cast<F16, CF16>(ca, "in_f16", "out_cf16");
+    bind_cast<F16, F16>(ca, "in_f16", "out_f16");
+    bind_cast<F32, I32>(ca, "in_f32", "out_i32");
     bind_cast<F32, F16>(ca, "in_f32", "out_f16");
-    bind_cast<CI8, CI8>(ca, "in_ci8", "out_ci8");
+    bind_cast<F32, CF32>(ca, "in_f32", "out_cf32");
+    bind_cast<F32, CF16>(ca, "in_f32", "out_cf16");
+    bind_cast<F32, F32>(ca, "in_f32", "out_f32");
     bind_cast<CI8, CF32>(ca, "in_ci8", "out_cf32");
     bind_cast<CI8, CF16>(ca, "in_ci8", "out_cf16");
+    bind_cast<CI8, CI8>(ca, "in_ci8", "out_ci8");
+    bind_cast<CF32, F16>(ca, "in_cf32", "out_f16");
+    bind_cast<CF32, F32>(ca, "in_cf32", "out_f32");
     bind_cast<CF32, CF16>(ca, "in_cf32", "out_cf16");
     bind_cast<CF32, CF32>(ca, "in_cf32", "out_cf32");
+    bind_cast<CF16, F16>(ca, "in_cf16", "out_f16");
+    bind_cast<CF16, F32>(ca, "in_cf16", "out_f32");
     bind_cast<CF16, CF32>(ca, "in_cf16", "out_cf32");
     bind_cast<CF16, CF16>(ca, "in_cf16", "out_cf16");
 

@@ -26,6 +26,20 @@ def test_reference_module_tree():
     assert hasattr(mb.tel_ata.in_ci8.out_f32, "mod")
 
 
+REFERENCE_CAST_PAIRS = [("i8", "i8"), ("i8", "f32"), ("f16", "f32"), ("f16", "cf32"), ("f16", "cf16"),
+                        ("f16", "f16"), ("f32", "i32"), ("f32", "f16"), ("f32", "cf32"), ("f32", "cf16"),
+                        ("f32", "f32"), ("ci8", "cf32"), ("ci8", "cf16"), ("ci8", "ci8"), ("cf32", "f16"),
+                        ("cf32", "f32"), ("cf32", "cf16"), ("cf32", "cf32"), ("cf16", "f16"),
+                        ("cf16", "f32"), ("cf16", "cf32"), ("cf16", "cf16")]
+
+
+def test_cast_module_tree_has_the_reference_pairs():
+    # python/blade/ext_cast.cc:43-75 registers these 22 (input, output) pairs.
+    import blade_amd._cast_impl as ca
+    for it, ot in REFERENCE_CAST_PAIRS:
+        assert hasattr(getattr(getattr(ca, "in_" + it), "out_" + ot), "mod"), (it, ot)
+
+
 def test_config_kwargs_match_reference():
     import blade_amd._beamformer_impl as bf
     import blade_amd._channelizer_impl as ch

@@ -239,6 +239,41 @@ def test_cast_module():
     assert np.array_equal(host_output.as_numpy(), host_input.as_numpy().astype(np.float32))
 
 
+@bl.runner
+class CastF32I32Pipeline:
+    def __init__(self, shape):
+        self.input.buf = bl.array_tensor(shape, dtype=bl.f32)
+        self.output.buf = bl.array_tensor(shape, dtype=bl.i32)
+        self.module.cast = bl.module(bl.cast, {}, self.input.buf, it=bl.f32, ot=bl.i32)
+
+    def transfer_in(self, buf):
+        self.copy(self.input.buf, buf)
+
+    def transfer_out(self, buf):
+        self.copy(self.output.buf, self.module.cast.get_output())
+        self.copy(buf, self.output.buf)
+
+
+def test_cast_module_f32_to_i32():
+    # Cast<F32, I32> (src/modules/cast/base.cc:68-100): conversion toward zero.
+    shape = (2, 16, 1000, 2)
+    v = (np.random.default_rng(8).normal(size=shape) * 1e4).astype(np.float32)
+    host_input = bl.array_tensor(shape, dtype=bl.f32, device=bl.cpu)
+    host_output = bl.array_tensor(shape, dtype=bl.i32, device=bl.cpu)
+    fill(host_input, v)
+    CastF32I32Pipeline(shape)(host_input, host_output)
+    assert host_output.as_numpy().dtype == np.int32
+    assert np.array_equal(host_output.as_numpy(), O.cast(v, "f32", "i32"))
+
+
+@pytest.mark.parametrize("it,ot", [("f32", "cf32"), ("f16", "cf16"), ("cf32", "f32"), ("cf16", "f16")])
+def test_cast_module_rejects_real_complex_pairs(it, ot):
+    # src/modules/cast/base.cc:39-43: the module throws at construction.
+    x = bl.array_tensor((1, 2, 8, 2), dtype=getattr(bl, it))
+    with pytest.raises(RuntimeError):
+        bl.module(bl.cast, {}, x, it=getattr(bl, it), ot=getattr(bl, ot))
+
+
 # ---------------------------------------------------------------- gather.py
 
 @bl.runner
