@@ -671,6 +671,9 @@ PYBIND11_MODULE(_blade_ext, m) {
     bind_gather<CF16, CF16>(ga, "in_cf16", "out_cf16");
     bind_gather<CF32, CF32>(ga, "in_cf32", "out_cf32");
     bind_gather<F32, F32>(ga, "in_f32", "out_f32");
+    // The reference registers the F32 gather as in_f64 / out_f64
+    // (python/blade/ext_gather.cc:50); both paths resolve to it here.
+    ga.def_submodule("in_f64").attr("out_f64") = ga.attr("in_f32").attr("out_f32");
 
     auto du = m.def_submodule("_duplicate_impl");
     bind_duplicate<CI8, CI8>(du, "in_ci8", "out_ci8");

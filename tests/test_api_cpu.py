@@ -40,6 +40,12 @@ def test_cast_module_tree_has_the_reference_pairs():
         assert hasattr(getattr(getattr(ca, "in_" + it), "out_" + ot), "mod"), (it, ot)
 
 
+def test_gather_f32_under_the_reference_f64_path():
+    # python/blade/ext_gather.cc:50 registers Gather<F32, F32> as in_f64/out_f64.
+    import blade_amd._gather_impl as ga
+    assert ga.in_f64.out_f64.mod is ga.in_f32.out_f32.mod
+
+
 def test_config_kwargs_match_reference():
     import blade_amd._beamformer_impl as bf
     import blade_amd._channelizer_impl as ch
