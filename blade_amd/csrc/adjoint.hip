@@ -32,6 +32,14 @@ constexpr int ADJ_TC = 128;        // time samples per LDS tile
 constexpr int ADJ_WAVES = 4;
 constexpr int ADJ_MAXAT = 8;       // antenna tiles of 16: A <= 128
 constexpr int ADJ_MAXB = 64;       // beams of the fused loss
+constexpr int ADJ_GYS = 17;        // s_gy row stride: 16 values + 1 word of padding
+constexpr int ADJ_BATCH = 8;       // global loads in flight per thread while staging (16 costs a wave of occupancy)
+
+// Row stride (words) of the LDS voltage tile s_x[t][antenna]: the padded
+// antenna count plus one, so the 16 lanes of a group, which read 16
+// consecutive times of one antenna, hit 16 different banks (a stride of a
+// multiple of 16 words would put them in two).
+__host__ __device__ constexpr int adj_xs(int apad) { return apad + 1; }
 
 struct AdjArgs {
     const uint32_t* x;       // [A,F,T] dwords: CI8 (re0, im0, re1, im1)
@@ -58,9 +66,10 @@ template <bool FUSED>
 __global__ __launch_bounds__(256) void adjoint_kernel(AdjArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int apad = a.apad;
-    uint32_t* s_x = reinterpret_cast<uint32_t*>(lds_raw);              // [TC][apad]
-    float* s_gy = reinterpret_cast<float*>(s_x + ADJ_TC * apad);        // [TC][16]
-    float* s_G = s_gy + ADJ_TC * 16;                                    // [2][16][apad]
+    const int xs = adj_xs(apad);
+    uint32_t* s_x = reinterpret_cast<uint32_t*>(lds_raw);              // [TC][xs]
+    float* s_gy = reinterpret_cast<float*>(s_x + ADJ_TC * xs);          // [TC][ADJ_GYS]
+    float* s_G = s_gy + ADJ_TC * ADJ_GYS;                               // [2][16][apad]
     float* s_w = s_G + 2 * 16 * apad;                                   // [2][nsteps][64]
     __shared__ float s_loss;
 
@@ -83,18 +92,31 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjArgs a) {
     if constexpr (FUSED) {
         // Forward A operand of beams b0..b0+7, both pols:
         // [p][step][g][row], step = 2 * antenna group + component.
-        for (int e = tid; e < 2 * nsteps * 64; e += blockDim.x) {
-            const int p = e / (nsteps * 64);
-            const int r = e - p * nsteps * 64;
-            const int st = r >> 6, kg = (r >> 4) & 3, row = r & 15;
-            const int an = (st >> 1) * 4 + kg, c = st & 1;
-            const int b = b0 + (row >> 1), co = row & 1;
-            float v = 0.f;
-            if (b < B && an < A) {
-                const float2 w = a.w[(((size_t)b * A + an) * F + f) * 2 + p];
-                v = (co == 0) ? (c == 0 ? w.x : -w.y) : (c == 0 ? w.y : w.x);
+        // Loads in batches of ADJ_BATCH per thread, all issued before any
+        // is used, so a batch costs one memory round trip.
+        const int n_w = 2 * nsteps * 64;
+        for (int e0 = tid; e0 < n_w; e0 += ADJ_BATCH * ADJ_WAVES * 64) {
+            float2 wv[ADJ_BATCH];
+#pragma unroll
+            for (int k = 0; k < ADJ_BATCH; k++) {
+                const int e = e0 + k * ADJ_WAVES * 64;
+                const int p = e / (nsteps * 64);
+                const int r = e - p * nsteps * 64;
+                const int st = r >> 6, kg = (r >> 4) & 3, row = r & 15;
+                const int an = (st >> 1) * 4 + kg, b = b0 + (row >> 1);
+                wv[k] = (e < n_w && b < B && an < A) ? a.w[(((size_t)b * A + an) * F + f) * 2 + p]
+                                                      : make_float2(0.f, 0.f);
             }
-            s_w[e] = v;
+#pragma unroll
+            for (int k = 0; k < ADJ_BATCH; k++) {
+                const int e = e0 + k * ADJ_WAVES * 64;
+                if (e < n_w) {
+                    const int r = e % (nsteps * 64);
+                    const int c = (r >> 6) & 1, co = r & 1;
+                    const float2 w = wv[k];
+                    s_w[e] = (co == 0) ? (c == 0 ? w.x : -w.y) : (c == 0 ? w.y : w.x);
+                }
+            }
         }
     }
 
@@ -108,12 +130,21 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjArgs a) {
     for (int t0 = t_begin; t0 < t_end; t0 += ADJ_TC) {
         __syncthreads();   // the previous tile's readers are done
         // Voltages of every antenna for this tile, s_x[t][a] (zero-padded).
-        for (int e = tid; e < ADJ_TC * apad; e += blockDim.x) {
-            const int an = e / ADJ_TC, t = e - an * ADJ_TC;
-            const int tt = t0 + t;
-            uint32_t v = 0u;
-            if (an < A && tt < t_end) v = a.x[((size_t)an * F + f) * T + tt];
-            s_x[t * apad + an] = v;
+        const int n_x = ADJ_TC * apad;
+        for (int e0 = tid; e0 < n_x; e0 += ADJ_BATCH * ADJ_WAVES * 64) {
+            uint32_t v[ADJ_BATCH];
+#pragma unroll
+            for (int k = 0; k < ADJ_BATCH; k++) {
+                const int e = e0 + k * ADJ_WAVES * 64;
+                const int an = e / ADJ_TC, tt = t0 + (e - an * ADJ_TC);
+                v[k] = (e < n_x && an < A && tt < t_end) ? a.x[((size_t)an * F + f) * T + tt] : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < ADJ_BATCH; k++) {
+                const int e = e0 + k * ADJ_WAVES * 64;
+                const int an = e / ADJ_TC, t = e - an * ADJ_TC;
+                if (e < n_x) s_x[t * xs + an] = v[k];
+            }
         }
         __syncthreads();
 
@@ -129,7 +160,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjArgs a) {
 #pragma unroll
                     for (int n = 0; n < 2; n++) {
                         const int t = wave * 32 + n * 16 + i;
-                        const float xv = ci8_component(s_x[t * apad + an], 2 * p + c);
+                        const float xv = ci8_component(s_x[t * xs + an], 2 * p + c);
                         acc[n] = adj_mfma(wv, xv, acc[n]);
                     }
                 }
@@ -142,17 +173,18 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjArgs a) {
                         const int b = b0 + 2 * g + (r >> 1);
                         const float lam = b < B ? a.lambda[b] : 0.f;
                         loss += lam * acc[n][r] * acc[n][r];
-                        s_gy[t * 16 + 4 * g + r] = -lam * acc[n][r];
+                        s_gy[t * ADJ_GYS + 4 * g + r] = -lam * acc[n][r];
                     }
                 }
             } else {
-                for (int e = tid; e < ADJ_TC * 8; e += blockDim.x) {
+#pragma unroll
+                for (int e = tid; e < ADJ_TC * 8; e += ADJ_WAVES * 64) {
                     const int bb = e / ADJ_TC, t = e - bb * ADJ_TC;
                     const int b = b0 + bb, tt = t0 + t;
                     float2 v = make_float2(0.f, 0.f);
                     if (b < B && tt < t_end) v = a.gy[(((size_t)b * F + f) * T + tt) * 2 + p];
-                    s_gy[t * 16 + 2 * bb] = v.x;
-                    s_gy[t * 16 + 2 * bb + 1] = v.y;
+                    s_gy[t * ADJ_GYS + 2 * bb] = v.x;
+                    s_gy[t * ADJ_GYS + 2 * bb + 1] = v.y;
                 }
             }
             __syncthreads();
@@ -161,14 +193,14 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjArgs a) {
 #pragma unroll 2
             for (int t4 = 0; t4 < 8; t4++) {
                 const int t = wave * 32 + t4 * 4 + g;
-                const float gr = s_gy[t * 16 + 2 * (i >> 1)];
-                const float gi = s_gy[t * 16 + 2 * (i >> 1) + 1];
+                const float gr = s_gy[t * ADJ_GYS + 2 * (i >> 1)];
+                const float gi = s_gy[t * ADJ_GYS + 2 * (i >> 1) + 1];
                 const float a0 = (i & 1) == 0 ? gr : gi;     // multiplies x_r
                 const float a1 = (i & 1) == 0 ? gi : -gr;    // multiplies x_i
 #pragma unroll
                 for (int at = 0; at < ADJ_MAXAT; at++) {
                     if (at < a.n_at) {
-                        const uint32_t d = s_x[t * apad + at * 16 + i];
+                        const uint32_t d = s_x[t * xs + at * 16 + i];
                         acc2[p][at] = adj_mfma(a0, ci8_component(d, 2 * p), acc2[p][at]);
                         acc2[p][at] = adj_mfma(a1, ci8_component(d, 2 * p + 1), acc2[p][at]);
                     }
@@ -285,7 +317,7 @@ static int run_adjoint(bool fused, const void* input, const void* phasors, const
     a.partial = reinterpret_cast<float*>(ws);
     a.loss_part = a.partial + part_floats;
 
-    const size_t lds = ((size_t)ADJ_TC * a.apad + ADJ_TC * 16 + 2 * 16 * a.apad +
+    const size_t lds = ((size_t)ADJ_TC * adj_xs(a.apad) + ADJ_TC * ADJ_GYS + 2 * 16 * a.apad +
                         (fused ? 2 * (size_t)a.nsteps * 64 : 0)) * 4;
     const void* fn = fused ? reinterpret_cast<const void*>(&adjoint_kernel<true>)
                            : reinterpret_cast<const void*>(&adjoint_kernel<false>);
