@@ -1367,10 +1367,10 @@ extern "C" int blade_channelizer_detector(const void* input, void* output, uint6
     if (out_pols != 1 && out_pols != 4) return fail("channelizer-detector: output polarizations must be 1 or 4");
     const uint64_t S = T / rate;
     if (integration == 0) return fail("channelizer-detector: integration size should be more than zero");
+    if (2 * rate > (uint64_t)kMaxLdsComplex) return fail("channelizer-detector: rate above 8192 is not supported");
+    if (A * F * T == 0) return BLADE_SUCCESS;     // before S is used as a divisor
     if (integration <= S ? (S % integration != 0) : (integration % S != 0))
         return fail("channelizer-detector: integration must divide, or be a multiple of, the spectra per block");
-    if (2 * rate > (uint64_t)kMaxLdsComplex) return fail("channelizer-detector: rate above 8192 is not supported");
-    if (A * F * T == 0) return BLADE_SUCCESS;
     if (!input || !output) return fail("channelizer-detector: null pointer");
     if ((uintptr_t)input % 16 || (uintptr_t)output % 16) return fail("channelizer-detector: misaligned buffer");
     const int C = (int)rate;

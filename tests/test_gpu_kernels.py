@@ -138,6 +138,27 @@ def test_beamform_chain_caller_workspace(precision):
                       ct.c_void_p(ws.data_ptr()), nbytes - 1, ops.stream_handle())
 
 
+def test_zero_size_calls_are_no_ops():
+    # Empty inputs return success without touching the (null) buffers.
+    import ctypes as ct
+    from blade_amd import _lib
+    null = ct.c_void_p(None)
+    s = ops.stream_handle()
+    cf32 = _lib.TYPE_CODES["cf32"]
+    _lib.call("blade_cast", _lib.TYPE_CODES["ci8"], cf32, null, null, 0, s)
+    _lib.call("blade_cast", cf32, cf32, null, null, 0, s)
+    _lib.call("blade_polarizer", cf32, null, null, 0, s)
+    _lib.call("blade_gather", null, null, 0, 16, 4, 1, s)
+    _lib.call("blade_permutation", null, null, 8, (ct.c_uint64 * 4)(2, 0, 3, 2),
+              (ct.c_uint64 * 4)(0, 2, 3, 1), s)
+    _lib.call("blade_detector", null, null, 2, 3, 0, 4, 1, 0, s)
+    _lib.call("blade_channelizer", null, null, 2, 3, 0, 2, 4, s)
+    # T = 0 leaves no spectra per block: the integration check must not
+    # divide by that count.
+    _lib.call("blade_channelizer_detector", null, null, 2, 3, 0, 8, 4, 1, 0, s)
+    torch.cuda.synchronize()
+
+
 def test_beamformer_rejects_too_many_antennas_for_lds():
     # 16 beams x 400 antennas of packed phasors exceed the LDS stage.
     x = torch.zeros((400, 1, 256, 2, 2), dtype=torch.int8, device="cuda")
