@@ -25,6 +25,10 @@ namespace Modules::Beamformer {
 template <typename IT, typename OT>
 Generic<IT, OT>::Generic(const Config& config, const Input& input, const Stream&)
     : config(config), input(input) {
+    if (config.blockSize == 0) {   // the reference divides by it unchecked (generic.cc:17)
+        BL_FATAL("The block size should be more than zero.");
+        throw Result::ERROR;
+    }
     if (getInputBuffer().shape().numberOfTimeSamples() % config.blockSize != 0) {
         BL_FATAL("Number of time samples ({}) isn't divisable by the block size ({}).",
                  getInputBuffer().shape().numberOfTimeSamples(), config.blockSize);

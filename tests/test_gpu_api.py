@@ -68,6 +68,14 @@ def test_beamformer_module(shapes):
     assert O.norm_rel_err(host_output.as_numpy(), ref) <= TOL
 
 
+def test_beamformer_module_rejects_zero_block_size():
+    x = bl.array_tensor((2, 4, 512, 2), dtype=bl.cf32)
+    ph = bl.phasor_tensor((1, 2, 4, 1, 2), dtype=bl.cf32)
+    cfg = {"enable_incoherent_beam": False, "enable_incoherent_beam_sqrt": False, "block_size": 0}
+    with pytest.raises(RuntimeError):
+        bl.module(bl.beamformer, cfg, (x, ph), telescope=bl.ata)
+
+
 def test_beamformer_module_ci8_split_fp16():
     # The CI8 instantiation (widening fused into the kernel) with split FP16.
     A, F, T, B = 28, 8, 1024, 16
