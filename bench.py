@@ -715,6 +715,10 @@ def main():
     ap.add_argument("--kernel-only", choices=["f32", "f16x2"],
                     help="profiling mode: launch only this variant of the fused kernel, then a "
                          "plain 1 GiB device copy (blade_gather) that calibrates HBM counters")
+    ap.add_argument("--leg", choices=["batched", "mode_a", "calibration", "kernels", "ingest",
+                                      "reference_benchmarks", "mode_b_bundle", "mode_h_bundle",
+                                      "unfused_f32"],
+                    help="profiling mode: run only this extra leg and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by "
@@ -723,6 +727,13 @@ def main():
         sys.exit(2)
     if args.kernel_only:
         kernel_only(args.kernel_only, args.steps, args.warmup)
+        return
+    if args.leg:
+        legs = {"batched": batched_leg, "mode_a": mode_a_leg, "calibration": calibration_leg,
+                "kernels": kernels_leg, "ingest": ingest_leg,
+                "reference_benchmarks": reference_benchmarks_leg, "mode_b_bundle": mode_b_bundle_leg,
+                "mode_h_bundle": mode_h_bundle_leg, "unfused_f32": unfused_leg}
+        print(json.dumps({args.leg: legs[args.leg]()}))
         return
 
     res, rel_f16_vs_f32, bufs = gpu_leg(args.steps, args.warmup)
