@@ -34,6 +34,10 @@ except ImportError as _e:
         "`python -m blade_amd.build` (the HIP library is required): " + str(_e))
 
     def __getattr__(name):
+        # Submodules stay importable (``from blade_amd import build`` looks
+        # the name up here before it imports the submodule).
+        if name in ("build", "_lib", "ops", "autotune", "calibration"):
+            return importlib.import_module(f"{__name__}.{name}")
         raise _import_error
 
     _ext = None
