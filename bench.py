@@ -478,18 +478,16 @@ ATA20 = [(-2524041.5388905862, -4123587.965024342, 4147646.4222955606),
          (-2523824.598229116, -4123527.93080514, 4147833.98936114)]
 
 
-def mode_b_bundle_leg(blocks=10):
-    """The reference's Mode-B bundle benchmark configuration
-    (benchmarks/bundles/ata/mode-b/mode_b.hh:55-115): CI8 [20,192,8192,2] ->
-    delays and phasors for 8 beams + incoherent beam -> beamform -> Stokes I
-    -> F32 [9,192,8192,1], through a Runner with host buffers (copy in, the
-    bundle, copy out, synchronously per block).  Wall time per block."""
+def _mode_b_reference_config(bl, out_shape, beams, detector):
+    """The ModeB configuration of the reference's bundle benchmarks
+    (benchmarks/bundles/ata/mode-b/mode_b.hh:55-115, mode-bh/mode_bh.hh:146-196):
+    20 ATA antennas, unit calibrations, 6500.125 MHz, `beams` beams (the
+    first off the boresight) and the incoherent beam."""
     import math
-    import blade_amd as bl
-    in_shape, out_shape = (20, 192, 8192, 2), (9, 192, 8192, 1)
+    in_shape = (20, 192, 8192, 2)
     cal = bl.array_tensor((20, 192, 1, 2), dtype=bl.cf64, device=bl.cpu)
     np.copyto(cal.as_numpy(), np.ones((20, 192, 1, 2), dtype=np.complex128))
-    config = {
+    return {
         "input_shape": in_shape, "output_shape": out_shape,
         "pre_beamformer_channelizer_rate": 1,
         "phasor_observation_frequency_hz": 6500.125e6, "phasor_channel_bandwidth_hz": 0.5e6,
@@ -501,11 +499,162 @@ def mode_b_bundle_leg(blocks=10):
         "phasor_antenna_positions": [bl.xyz(*q) for q in ATA20],
         "phasor_antenna_calibrations": cal,
         "phasor_beam_coordinates": [bl.ra_dec(ra=0.63722, dec=1.07552424)] +
-                                   [bl.ra_dec(ra=0.64169, dec=1.079896295)] * 7,
+                                   [bl.ra_dec(ra=0.64169, dec=1.079896295)] * (beams - 1),
         "beamformer_incoherent_beam": True,
-        "detector_enable": True, "detector_integration_size": 1,
+        "detector_enable": detector, "detector_integration_size": 1,
         "detector_number_of_output_polarizations": 1,
-    }
+    }, cal
+
+
+def _drive_async(bl, p, n_outputs, ins, outs):
+    """The reference benchmarks' run loop (benchmarks/bundles/ata/mode-b/
+    mode_b.hh:127-154): enqueue with the two host buffers in turn, dequeue,
+    until `n_outputs` outputs came back.  Returns the wall time in seconds."""
+    state = {"in": 0, "out": 0, "done": 0}
+
+    def input_callback():
+        p.transfer_in(*ins[state["in"] % 2])
+        state["in"] += 1
+        return bl.result.success
+
+    def output_callback():
+        p.transfer_out(*outs[state["out"] % 2])
+        state["out"] += 1
+        return bl.result.success
+
+    def callback(input_id, output_id, did_output):
+        if did_output:
+            state["done"] += 1
+        return bl.result.success
+
+    t0 = time.perf_counter()
+    while state["done"] < n_outputs:
+        p.enqueue(input_callback, output_callback, state["in"], state["out"])
+        p.dequeue(callback)
+    return time.perf_counter() - t0
+
+
+def _mode_b_pipeline(bl, config, in_shape, out_shape):
+    @bl.runner
+    class ModeB:
+        def __init__(self):
+            self.input.dut = bl.tensor(1, dtype=bl.f64, device=bl.cpu)
+            self.input.date = bl.tensor(1, dtype=bl.f64, device=bl.cpu)
+            self.input.buffer = bl.array_tensor(in_shape, dtype=bl.ci8)
+            self.output.buffer = bl.array_tensor(out_shape, dtype=bl.f32)
+            self.module.mode_b = bl.module(bl.modeb, config,
+                                           (self.input.dut, self.input.date, self.input.buffer),
+                                           it=bl.ci8, ot=bl.f32, telescope=bl.ata)
+
+        def transfer_in(self, dut, date, buffer):
+            self.copy(self.input.dut, dut)
+            self.copy(self.input.date, date)
+            self.copy(self.input.buffer, buffer)
+
+        def transfer_out(self, buffer):
+            self.copy(self.output.buffer, self.module.mode_b.get_output())
+            self.copy(buffer, self.output.buffer)
+
+    return ModeB()
+
+
+def _host_inputs(bl, in_shape, seed, count=2):
+    ins = []
+    rng = np.random.default_rng(seed)
+    for k in range(count):
+        dut = bl.tensor(1, dtype=bl.f64, device=bl.cpu)
+        date = bl.tensor(1, dtype=bl.f64, device=bl.cpu)
+        buf = bl.array_tensor(in_shape, dtype=bl.ci8, device=bl.cpu)
+        np.copyto(buf.as_numpy(), np.clip(np.round(rng.normal(size=in_shape + (2,)) * 16), -127, 127)
+                  .astype(np.int8))
+        dut.as_numpy()[0] = 0.0
+        date.as_numpy()[0] = (1649366473.0 + k * 0.016384) / 86400 + 2440587.5
+        ins.append((dut, date, buf))
+    return ins
+
+
+def mode_b_bundle_async_leg(blocks=24):
+    """The reference's Mode-B bundle benchmark as it runs there
+    (benchmarks/bundles/ata/mode-b/mode_b.hh:55-154): CI8 [20,192,8192,2] ->
+    8 beams + incoherent -> Stokes I -> F32 [9,192,8192,1] through the
+    Runner's enqueue/dequeue on two streams with two host buffers each way,
+    so block N+1's copy in overlaps block N's compute and copy out.  Wall
+    time per block over `blocks` blocks after two."""
+    import blade_amd as bl
+    in_shape, out_shape = (20, 192, 8192, 2), (9, 192, 8192, 1)
+    config, _cal = _mode_b_reference_config(bl, out_shape, 8, True)
+    p = _mode_b_pipeline(bl, config, in_shape, out_shape)
+    ins = _host_inputs(bl, in_shape, 3)
+    outs = [(bl.array_tensor(out_shape, dtype=bl.f32, device=bl.cpu),) for _ in range(2)]
+    _drive_async(bl, p, 2, ins, outs)
+    dt = _drive_async(bl, p, blocks, ins, outs)
+    ms = dt * 1e3 / blocks
+    return {"workload": "reference Mode-B benchmark through enqueue/dequeue: CI8 [20,192,8192,2] -> "
+                        "8 beams + incoherent, Stokes I -> F32 [9,192,8192,1], two host buffers each way",
+            "ms_per_block": round(ms, 3), "realtime_factor": round(16.384 / ms, 2),
+            "blocks": blocks, "finite": bool(np.isfinite(outs[0][0].as_numpy()).all())}
+
+
+def mode_bh_bundle_leg(outputs=3):
+    """The reference's Mode-BH bundle benchmark (benchmarks/bundles/ata/
+    mode-bh/mode_bh.hh:41-196): CI8 [20,192,8192,2] -> Mode B (2 beams +
+    incoherent, CF32 beams) -> Gather x8 along time -> Mode H at rate 65536
+    (Stokes I) -> F32 [3,12582912,1,1], through enqueue/dequeue with two host
+    buffers each way; eight Mode-B blocks per output.  Wall time per output
+    over `outputs` outputs after one."""
+    import blade_amd as bl
+    in_shape = (20, 192, 8192, 2)
+    g_in, g_out = (3, 192, 8192, 2), (3, 192, 65536, 2)
+    out_shape = (3, 192 * 65536, 1, 1)
+    config_b, _cal = _mode_b_reference_config(bl, g_in, 2, False)
+    config_h = {"input_shape": g_out, "output_shape": out_shape, "polarizer_convert_to_circular": False,
+                "detector_integration_size": 1, "detector_number_of_output_polarizations": 1}
+
+    @bl.runner
+    class ModeBH:
+        def __init__(self):
+            self.input.dut = bl.tensor(1, dtype=bl.f64, device=bl.cpu)
+            self.input.date = bl.tensor(1, dtype=bl.f64, device=bl.cpu)
+            self.input.buffer = bl.array_tensor(in_shape, dtype=bl.ci8)
+            self.output.buffer = bl.array_tensor(out_shape, dtype=bl.f32)
+            inp = (self.input.dut, self.input.date, self.input.buffer)
+            self.module.mode_b = bl.module(bl.modeb, config_b, inp, it=bl.ci8, ot=bl.cf32, telescope=bl.ata)
+            self.module.gather = bl.module(bl.gather, {"axis": 2, "multiplier": 8},
+                                           self.module.mode_b.get_output(), ot=bl.cf32)
+            self.module.mode_h = bl.module(bl.modeh, config_h, self.module.gather.get_output(),
+                                           ot=bl.f32)
+
+        def transfer_in(self, dut, date, buffer):
+            self.copy(self.input.dut, dut)
+            self.copy(self.input.date, date)
+            self.copy(self.input.buffer, buffer)
+
+        def transfer_out(self, buffer):
+            self.copy(self.output.buffer, self.module.mode_h.get_output())
+            self.copy(buffer, self.output.buffer)
+
+    p = ModeBH()
+    ins = _host_inputs(bl, in_shape, 4)
+    outs = [(bl.array_tensor(out_shape, dtype=bl.f32, device=bl.cpu),) for _ in range(2)]
+    _drive_async(bl, p, 1, ins, outs)
+    dt = _drive_async(bl, p, outputs, ins, outs)
+    ms = dt * 1e3 / outputs
+    return {"workload": "reference Mode-BH benchmark through enqueue/dequeue: CI8 [20,192,8192,2] -> "
+                        "2 beams + incoherent -> gather x8 -> FFT 65536 -> Stokes I -> F32 "
+                        "[3,12582912,1,1]; 8 input blocks (131 ms of sky) per output",
+            "ms_per_output": round(ms, 3), "realtime_factor": round(8 * 16.384 / ms, 2),
+            "outputs": outputs, "finite": bool(np.isfinite(outs[0][0].as_numpy()).all())}
+
+
+def mode_b_bundle_leg(blocks=10):
+    """The reference's Mode-B bundle benchmark configuration
+    (benchmarks/bundles/ata/mode-b/mode_b.hh:55-115): CI8 [20,192,8192,2] ->
+    delays and phasors for 8 beams + incoherent beam -> beamform -> Stokes I
+    -> F32 [9,192,8192,1], through a Runner with host buffers (copy in, the
+    bundle, copy out, synchronously per block).  Wall time per block."""
+    import blade_amd as bl
+    in_shape, out_shape = (20, 192, 8192, 2), (9, 192, 8192, 1)
+    config, _cal = _mode_b_reference_config(bl, out_shape, 8, True)
 
     @bl.runner
     class ModeB:
@@ -717,7 +866,7 @@ def main():
                          "plain 1 GiB device copy (blade_gather) that calibrates HBM counters")
     ap.add_argument("--leg", choices=["batched", "mode_a", "calibration", "kernels", "ingest",
                                       "reference_benchmarks", "mode_b_bundle", "mode_h_bundle",
-                                      "unfused_f32"],
+                                      "mode_b_bundle_async", "mode_bh_bundle", "unfused_f32"],
                     help="profiling mode: run only this extra leg and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -732,7 +881,8 @@ def main():
         legs = {"batched": batched_leg, "mode_a": mode_a_leg, "calibration": calibration_leg,
                 "kernels": kernels_leg, "ingest": ingest_leg,
                 "reference_benchmarks": reference_benchmarks_leg, "mode_b_bundle": mode_b_bundle_leg,
-                "mode_h_bundle": mode_h_bundle_leg, "unfused_f32": unfused_leg}
+                "mode_h_bundle": mode_h_bundle_leg, "mode_b_bundle_async": mode_b_bundle_async_leg,
+                "mode_bh_bundle": mode_bh_bundle_leg, "unfused_f32": unfused_leg}
         print(json.dumps({args.leg: legs[args.leg]()}))
         return
 
@@ -744,18 +894,21 @@ def main():
     achieved_gbs = bytes_ / t_k / 1e9
     t_step = f32["step_ms"] * 1e-3
     value = B * F * T * P / t_step / 1e6
-    # Report the bound whose lower-bound time is larger.  With 16 beams per
-    # wavefront the FP32 kernel forms three real products per complex one,
-    # so its matrix cores execute 6 of the 8 algorithmic flops of a complex
-    # multiply-add: the HBM floor is the higher one.
-    t_hbm = bytes_ / (HBM_PEAK_GBS * 1e9)
-    t_mfma = 0.75 * flops / (FP32_PEAK_TFLOPS * 1e12)
-    if t_mfma >= t_hbm:
+    # The fraction reported is the larger of the HBM fraction (algorithmic
+    # bytes) and the matrix-core fraction (algorithmic flops, 8 per complex
+    # multiply-add, against the FP32 matrix peak).  With 16 beams per
+    # wavefront the kernel forms three real products per complex one, so the
+    # matrix cores execute 6 of those 8 flops: `executed_mfma_frac`.
+    hbm_frac = achieved_gbs / HBM_PEAK_GBS
+    mfma_frac = achieved_tflops / FP32_PEAK_TFLOPS
+    if mfma_frac >= hbm_frac:
         roof = {"bound": "mfma", "achieved": round(achieved_tflops, 2), "peak": FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4)}
+                "unit": "TFLOP/s", "frac": round(mfma_frac, 4)}
     else:
         roof = {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 4)}
+                "unit": "GB/s", "frac": round(hbm_frac, 4)}
+    roof["mfma_frac"] = round(mfma_frac, 4)
+    roof["executed_mfma_frac"] = round(0.75 * mfma_frac, 4)
     roof["traffic"] = _traffic()
     roof["kernel"] = "beamform_kernel<4,2,true,false,4,true,12> (NT=4, MW=2, CI8, FP32 MFMA with three real products per complex one, Mode-B epilogue: FFT-4 + polarize; phasors packed into LDS by each workgroup)"
     roof["kernel_ms"] = round(f32["kernel_ms"], 5)
@@ -800,6 +953,8 @@ def main():
         line["reference_benchmarks"] = _extra(reference_benchmarks_leg)
         line["mode_b_bundle"] = _extra(mode_b_bundle_leg)
         line["mode_h_bundle"] = _extra(mode_h_bundle_leg)
+        line["mode_b_bundle_async"] = _extra(mode_b_bundle_async_leg)
+        line["mode_bh_bundle"] = _extra(mode_bh_bundle_leg)
     if not args.no_cpu:
         x, w, outs = bufs
         cpu, errs = cpu_leg(x, w, outs)

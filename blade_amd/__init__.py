@@ -36,7 +36,7 @@ except ImportError as _e:
     def __getattr__(name):
         # Submodules stay importable (``from blade_amd import build`` looks
         # the name up here before it imports the submodule).
-        if name in ("build", "_lib", "ops", "autotune", "calibration"):
+        if name in ("build", "_lib", "ops", "autotune", "calibration", "delays"):
             return importlib.import_module(f"{__name__}.{name}")
         raise _import_error
 

@@ -41,12 +41,13 @@ SIGNATURES = {
     "blade_beamformer_stokes_i_grad": [_p, _p, _p, _p, _p, _u64, _u64, _u64, _u64, _u64, _p],
     "blade_channelizer": [_p, _p, _u64, _u64, _u64, _u64, _u64, _p],
     "blade_channelizer_plan": [_u64, _p],
-    "blade_channelizer_detector": [_p, _p, _u64, _u64, _u64, _u64, _u64, _u64, _i, _p],
+    "blade_channelizer_detector": [_p, _p, _u64, _u64, _u64, _u64, _u64, _u64, _i, _i, _p],
     "blade_polarizer": [_i, _p, _p, _u64, _p],
     "blade_detector": [_p, _p, _u64, _u64, _u64, _u64, _u64, _i, _p],
     "blade_gather": [_p, _p, _u64, _u64, _u64, _u64, _p],
     "blade_permutation": [_p, _p, _u64, ct.POINTER(_u64), ct.POINTER(_u64), _p],
     "blade_phasor_ata": [_p, _p, _p, _i, _u64, _u64, _u64, _u64, _d, _d, _d, _u64, _p],
+    "blade_ata_delays": [_p, _u64, _u64, _p, _p, _p, _u64, _d, _d, _p],
 }
 _RESTYPES = {"blade_amd_version": ct.c_char_p, "blade_amd_last_error": ct.c_char_p,
              "blade_beamformer_workspace_size": ct.c_uint64,

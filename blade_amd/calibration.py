@@ -92,11 +92,14 @@ def steer_phasors(w: torch.Tensor, gains: torch.Tensor | None, delays: torch.Ten
 
 def calibrate(x: torch.Tensor, w: torch.Tensor, steps: int = 200, lr: float = 0.05,
               beam_weights=None, fit_gains: bool = True, fit_delays: bool = False,
-              freqs_hz: torch.Tensor | None = None, delay_lr: float = 1e-10):
+              freqs_hz: torch.Tensor | None = None, delay_lr: float = 1e-10, graph: bool = True):
     """Adam on unit-modulus gains [A,F,P] (and delays [A]) maximising beam power.
 
     Returns ``(gains, delays, losses)``; the loss is normalised by its value at
-    the start so that its history is comparable across data sets.
+    the start so that its history is comparable across data sets.  The losses
+    stay on the device until the end (one read).  With ``graph`` the step
+    (steering, fused loss and gradient, chain rule, Adam) is captured once in
+    a CUDA graph after a few eager steps on the same stream and replayed.
     """
     A, F, P = w.shape[1], w.shape[2], w.shape[4]
     params = []
@@ -109,16 +112,41 @@ def calibrate(x: torch.Tensor, w: torch.Tensor, steps: int = 200, lr: float = 0.
             raise ValueError("delay calibration needs the channel frequencies")
         delays = torch.zeros((A,), dtype=torch.float64, device=w.device, requires_grad=True)
         params.append({"params": [delays], "lr": delay_lr})
-    opt = torch.optim.Adam(params)
-    losses = []
-    ref = None
-    for _ in range(steps):
-        opt.zero_grad(set_to_none=True)
-        w_eff = steer_phasors(w, gains, delays, freqs_hz)
-        loss = stokes_i_power(x, w_eff, beam_weights)
-        if ref is None:
-            ref = abs(float(loss.detach()))
-        (loss / ref).backward()
+    # capturable Adam (step counts on the device) with or without a graph, so
+    # both run the same arithmetic
+    opt = torch.optim.Adam(params, capturable=True, foreach=True)
+    losses = torch.zeros(steps, dtype=torch.float64, device=w.device)
+    if steps == 0:
+        return gains, delays, []
+    with torch.no_grad():
+        ref = abs(float(stokes_i_power(x, steer_phasors(w, gains, delays, freqs_hz), beam_weights)))
+    inv = 1.0 / ref if ref > 0 else 1.0
+
+    def step():
+        loss = stokes_i_power(x, steer_phasors(w, gains, delays, freqs_hz), beam_weights)
+        (loss * inv).backward()
         opt.step()
-        losses.append(float(loss.detach()) / ref)
-    return gains, delays, losses
+        return loss.detach() * inv
+
+    if not graph:
+        for i in range(steps):
+            opt.zero_grad(set_to_none=True)
+            losses[i].copy_(step())
+        return gains, delays, losses.tolist()
+    side = torch.cuda.Stream(device=w.device)
+    side.wait_stream(torch.cuda.current_stream(w.device))
+    warm = min(3, steps)
+    with torch.cuda.stream(side):
+        for i in range(warm):
+            opt.zero_grad(set_to_none=True)
+            losses[i].copy_(step())
+    torch.cuda.current_stream(w.device).wait_stream(side)
+    if steps > warm:
+        g = torch.cuda.CUDAGraph()
+        opt.zero_grad(set_to_none=True)
+        with torch.cuda.graph(g, stream=side):
+            static_loss = step()
+        for i in range(warm, steps):
+            g.replay()
+            losses[i].copy_(static_loss)
+    return gains, delays, losses.tolist()

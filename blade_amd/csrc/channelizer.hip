@@ -8,6 +8,8 @@
 // once to FP32.
 #include "common.hh"
 
+#include <algorithm>
+#include <cmath>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -194,21 +196,24 @@ constexpr int groups_max() {
     return (kMaxLdsComplex / R + 1023) / 1024;
 }
 
+// One radix-R pass over PC / C sequences of C points in LDS, sequence p at
+// data + p * pstride (pstride = C by default).
 template <int R>
 __device__ __forceinline__ void stockham_pass(float2* __restrict__ data, int C, int PC, int Ns,
-                                              const float2* __restrict__ tw) {
+                                              const float2* __restrict__ tw, int pstride = 0) {
     constexpr int G = groups_max<R>();
     const int nthr = blockDim.x;
     const int per_pol = C / R;
     const int total = PC / R;
     const int twstride = C / (Ns * R);
+    if (pstride == 0) pstride = C;
     float2 v[G][R];
 #pragma unroll
     for (int g = 0; g < G; g++) {
         const int gi = threadIdx.x + g * nthr;
         if (gi < total) {
             const int p = gi / per_pol, j = gi - p * per_pol;
-            const float2* base = data + p * C;
+            const float2* base = data + p * pstride;
             const int jm = j % Ns;
 #pragma unroll
             for (int r = 0; r < R; r++) {
@@ -228,7 +233,7 @@ __device__ __forceinline__ void stockham_pass(float2* __restrict__ data, int C, 
         const int gi = threadIdx.x + g * nthr;
         if (gi < total) {
             const int p = gi / per_pol, j = gi - p * per_pol;
-            float2* base = data + p * C;
+            float2* base = data + p * pstride;
             const int jm = j % Ns;
             const int d = (j - jm) * R + jm;
 #pragma unroll
@@ -288,6 +293,110 @@ __global__ __launch_bounds__(1024) void chan_lds_kernel(const float2* __restrict
     }
 }
 
+// All passes of `rad` over PC / N sequences of N points at stride pstride.
+__device__ __forceinline__ void stockham_all(float2* data, int N, int PC, const float2* tw, const Radices& rad,
+                                             int pstride) {
+    int Ns = 1;
+    for (int i = 0; i < rad.n; i++) {
+        switch (rad.r[i]) {
+            case 2: stockham_pass<2>(data, N, PC, Ns, tw, pstride); break;
+            case 3: stockham_pass<3>(data, N, PC, Ns, tw, pstride); break;
+            case 4: stockham_pass<4>(data, N, PC, Ns, tw, pstride); break;
+            case 5: stockham_pass<5>(data, N, PC, Ns, tw, pstride); break;
+            case 7: stockham_pass<7>(data, N, PC, Ns, tw, pstride); break;
+            default: stockham_pass<8>(data, N, PC, Ns, tw, pstride); break;
+        }
+        Ns *= rad.r[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Rates past one workgroup's LDS (P * C > 16384): two passes through a
+// scratch buffer (the four-step FFT).  With C = N1 * N2, n = n1 + N1 n2 and
+// k = k2 + N2 k1:
+//   pass 1, each n1:  Y[n1, k2] = W_C^(n1 k2) sum_n2 x[n1 + N1 n2] W_N2^(n2 k2)
+//   pass 2, each k2:  X[k2 + N2 k1] = sum_n1 Y[n1, k2] W_N1^(n1 k1)
+// and the fftshift of X moves k1 by N1 / 2 (N1 even).  Scratch layout
+// [spectrum][pol][n1][k2].  A workgroup (1024 threads) transforms a tile of
+// `cols` consecutive n1 (pass 1) or k2 (pass 2) of one spectrum, both pols,
+// so every global access moves `cols` consecutive samples; the LDS sequences
+// are padded by one sample against bank conflicts.
+// ---------------------------------------------------------------------------
+
+template <int P>
+__global__ __launch_bounds__(1024) void chan_big_pass1(const float2* __restrict__ in, float2* __restrict__ y,
+                                                       int C, int N1, int N2, int cols, uint64_t n_spec,
+                                                       const float2* __restrict__ twN2,
+                                                       const float2* __restrict__ twC, Radices rad) {
+    extern __shared__ float2 data[];
+    const int stride = N2 + 1;
+    const int L = P * cols;
+    const uint64_t tiles = (uint64_t)(N1 / cols);
+    for (uint64_t it = blockIdx.x; it < n_spec * tiles; it += gridDim.x) {
+        const uint64_t sp = it / tiles;
+        const int n1_0 = (int)(it - sp * tiles) * cols;
+        const float2* src = in + sp * (uint64_t)C * P;
+        for (int e = threadIdx.x; e < cols * N2; e += blockDim.x) {
+            const int n2 = e / cols, c = e - n2 * cols;
+            const uint64_t n = (uint64_t)(n1_0 + c) + (uint64_t)N1 * n2;
+            if constexpr (P == 2) {
+                const float4 q = reinterpret_cast<const float4*>(src)[n];
+                data[c * stride + n2] = make_float2(q.x, q.y);
+                data[(cols + c) * stride + n2] = make_float2(q.z, q.w);
+            } else {
+                data[c * stride + n2] = src[n];
+            }
+        }
+        __syncthreads();
+        stockham_all(data, N2, L * N2, twN2, rad, stride);
+        for (int e = threadIdx.x; e < L * N2; e += blockDim.x) {
+            const int q = e / N2, k2 = e - q * N2;
+            const int pp = q / cols, c = q - pp * cols;
+            const uint64_t n1 = (uint64_t)(n1_0 + c);
+            const float2 w = twC[(n1 * (uint64_t)k2) % (uint64_t)C];
+            y[((sp * P + pp) * N1 + n1) * (uint64_t)N2 + k2] = cmul(data[q * stride + k2], w);
+        }
+        __syncthreads();
+    }
+}
+
+template <int P>
+__global__ __launch_bounds__(1024) void chan_big_pass2(const float2* __restrict__ y, float2* __restrict__ out,
+                                                       int C, int N1, int N2, int cols, uint64_t n_spec,
+                                                       uint64_t S, const float2* __restrict__ twN1, Radices rad) {
+    extern __shared__ float2 data[];
+    const int stride = N1 + 1;
+    const int L = P * cols;
+    const uint64_t tiles = (uint64_t)(N2 / cols);
+    const int half1 = N1 / 2;
+    for (uint64_t it = blockIdx.x; it < n_spec * tiles; it += gridDim.x) {
+        const uint64_t sp = it / tiles;
+        const int k2_0 = (int)(it - sp * tiles) * cols;
+        for (int e = threadIdx.x; e < L * N1; e += blockDim.x) {
+            const int c = e % cols, r = e / cols;
+            const int n1 = r % N1, pp = r / N1;
+            data[(pp * cols + c) * stride + n1] = y[((sp * P + pp) * N1 + n1) * (uint64_t)N2 + k2_0 + c];
+        }
+        __syncthreads();
+        stockham_all(data, N1, L * N1, twN1, rad, stride);
+        const uint64_t af = sp / S, s = sp - af * S;
+        for (int e = threadIdx.x; e < cols * N1; e += blockDim.x) {
+            const int k1 = e / cols, c = e - k1 * cols;
+            int k1o = k1 + half1;
+            if (k1o >= N1) k1o -= N1;
+            const uint64_t ko = (uint64_t)(k2_0 + c) + (uint64_t)N2 * k1o;
+            const uint64_t o = ((af * C + ko) * S + s) * P;
+            if constexpr (P == 2) {
+                const float2 X = data[c * stride + k1], Y = data[(cols + c) * stride + k1];
+                reinterpret_cast<float4*>(out + o)[0] = make_float4(X.x, X.y, Y.x, Y.y);
+            } else {
+                out[o] = data[c * stride + k1];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Channelizer with the detector fused on the output: one workgroup per
 // (a, f) walks its S = T/C spectra, transforms both pols in LDS and
 // integrates |X|^2+|Y|^2 (or XX*, YY*, Re XY*, Im XY*) of every fine channel
@@ -295,13 +404,33 @@ __global__ __launch_bounds__(1024) void chan_lds_kernel(const float2* __restrict
 //   out[a, f*C + k', s/N, q]   (N <= S, N | S)
 //   out[a, f*C + k', 0, q]    (+=) when N spans several blocks (accumulate).
 // The CF32 spectra never reach HBM.
+// Detector sums of (X, Y) turned into those of (L, R) = (X + jY, X - jY),
+// the polarizer (polarizer.cu:5-25) applied before the detector: sums are
+// linear in [XX*, YY*, Re XY*, Im XY*], so the map applies after integration.
+//   Stokes I doubles; |L|^2 = XX + YY + 2 Im, |R|^2 = XX + YY - 2 Im,
+//   Re LR* = XX - YY, Im LR* = 2 Re.
+template <int NPOL>
+__device__ inline void circular_sums(float (&a)[NPOL]) {
+    if constexpr (NPOL == 1) {
+        a[0] *= 2.f;
+    } else {
+        const float xx = a[0], yy = a[1], re = a[2], im = a[3];
+        a[0] = (xx + yy) + 2.f * im;
+        a[1] = (xx + yy) - 2.f * im;
+        a[2] = xx - yy;
+        a[3] = 2.f * re;
+    }
+}
+
 template <int NPOL, int KPT>
 __global__ __launch_bounds__(1024) void chan_detect_kernel(const float2* __restrict__ in,
                                                            float* __restrict__ out, int C,
                                                            int S, int N, uint64_t n_af,
                                                            const float2* __restrict__ tw,
-                                                           Radices rad, int accumulate) {
+                                                           Radices rad, int flags) {
     extern __shared__ float2 data[];
+    const int accumulate = flags & 1;
+    const bool circular = flags & 2;
     const int PC = 2 * C;
     const int groups = (N <= S) ? S / N : 1;
     const int half = C / 2;
@@ -349,6 +478,10 @@ __global__ __launch_bounds__(1024) void chan_detect_kernel(const float2* __restr
             const bool flush = (N <= S) ? ((s + 1) % N == 0) : (s == S - 1);
             if (flush) {
                 const int grp = (N <= S) ? s / N : 0;
+                if (circular) {
+#pragma unroll
+                    for (int j = 0; j < KPT; j++) circular_sums<NPOL>(acc[j]);
+                }
 #pragma unroll
                 for (int j = 0; j < KPT; j++) {
                     const int k = threadIdx.x + j * blockDim.x;
@@ -383,9 +516,11 @@ __global__ __launch_bounds__(1024) void chan_detect_kernel(const float2* __restr
 template <int MODE, int RL>
 __global__ __launch_bounds__(1024) void fft2_kernel(const float4* __restrict__ in, void* __restrict__ outv,
                                                     int C, int nmid, uint64_t S, uint64_t n_items,
-                                                    int N, int accumulate,
+                                                    int N, int flags,
                                                     const float2* __restrict__ tw) {
     extern __shared__ float2 data[];
+    const int accumulate = flags & 1;
+    const bool circular = flags & 2;
     constexpr int G = 8 / RL;                   // last-pass butterflies per thread and pol
     const int tps = C >> 3;                     // threads per spectrum
     const int spw = blockDim.x / tps;
@@ -490,6 +625,12 @@ __global__ __launch_bounds__(1024) void fft2_kernel(const float4* __restrict__ i
                 if (flush) {
                     const int grp = (N <= (int)S) ? s / N : 0;
                     float* out = static_cast<float*>(outv);
+                    if (circular) {
+#pragma unroll
+                        for (int g = 0; g < G; g++)
+#pragma unroll
+                            for (int r = 0; r < RL; r++) circular_sums<NPOL>(acc[g][r]);
+                    }
 #pragma unroll
                     for (int g = 0; g < G; g++)
 #pragma unroll
@@ -768,8 +909,10 @@ __device__ __forceinline__ void halves(float x, float& u, float& v) {
 // consecutive spectra of one (antenna, channel) land as whole lines.
 template <int LC, int NT, int MODE, int SM>
 __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__ in, void* __restrict__ outv,
-                                                      uint64_t S, uint64_t n_items, int N, int accumulate,
+                                                      uint64_t S, uint64_t n_items, int N, int flags,
                                                       const float2* __restrict__ tw, float4* __restrict__ sink) {
+    const int accumulate = flags & 1;
+    const bool circular = flags & 2;
     constexpr int C = 1 << LC;
     constexpr int TPS = C / 16;                        // threads per spectrum
     static_assert(NT % TPS == 0 && LC >= 8, "workgroup must hold whole spectra");
@@ -1057,6 +1200,10 @@ __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__
                 const bool flush = (N <= (int)S) ? ((s + 1) % N == 0) : (s == (int)S - 1);
                 if (flush) {
                     const int grp = (N <= (int)S) ? s / N : 0;
+                    if (circular) {
+#pragma unroll
+                        for (int i = 0; i < 16; i++) circular_sums<NPOL>(acc[i]);
+                    }
                     const unsigned gn = (unsigned)groups * NPOL;
                     float* const lb = valid ? static_cast<float*>(outv) + (item * C + ch0) * gn + grp * NPOL
                                             : reinterpret_cast<float*>(sink + threadIdx.x);
@@ -1233,6 +1380,75 @@ static bool factorize(int C, Radices* rad) {
     return c == 1;
 }
 
+// Split C = N1 * N2 for the two-pass FFT: N2 near sqrt(C), N1 even, each
+// tile of the passes within 16384 samples of LDS (with the padding sample);
+// tiles of up to 16 consecutive columns that divide the other length.
+struct BigPlan {
+    int N1, N2, cols1, cols2;
+    Radices r1, r2;
+};
+
+constexpr int kMaxBigRate = 1 << 20;     // the pass-1 twiddle table: 8 MiB
+
+static int big_cols(int n_other, int P, int len) {
+    int cols = 16;
+    while (cols > 1 && (P * cols * (len + 1) > kMaxLdsComplex || n_other % cols)) cols /= 2;
+    return P * cols * (len + 1) <= kMaxLdsComplex ? cols : 0;
+}
+
+static bool big_plan(int C, int P, BigPlan* pl) {
+    if (C > kMaxBigRate || C % 2) return false;
+    double best = 1e30;
+    bool found = false;
+    for (int n2 = 2; n2 < C; n2++) {
+        if (C % n2) continue;
+        const int n1 = C / n2;
+        if (n1 % 2) continue;
+        const int c1 = big_cols(n1, P, n2), c2 = big_cols(n2, P, n1);
+        if (!c1 || !c2) continue;
+        BigPlan t;
+        if (!factorize(n1, &t.r1) || !factorize(n2, &t.r2)) continue;
+        const double score = std::fabs(std::log((double)n2) - 0.5 * std::log((double)C)) - 0.01 * (c1 + c2);
+        if (score < best) {
+            best = score;
+            t.N1 = n1; t.N2 = n2; t.cols1 = c1; t.cols2 = c2;
+            *pl = t;
+            found = true;
+        }
+    }
+    return found;
+}
+
+template <int P>
+static int launch_big(const BigPlan& pl, int C, const float2* in, float2* out, uint64_t S, uint64_t n_spec,
+                      hipStream_t s) {
+    const float2 *twC = nullptr, *tw1 = nullptr, *tw2 = nullptr;
+    int rc = get_twiddles(C, s, &twC);
+    if (!rc) rc = get_twiddles(pl.N1, s, &tw1);
+    if (!rc) rc = get_twiddles(pl.N2, s, &tw2);
+    if (rc) return rc;
+    unsigned char* scratch = nullptr;
+    rc = library_workspace(WS_CHANNELIZER_BIG, sizeof(float2) * (size_t)n_spec * C * P, s, &scratch);
+    if (rc) return rc;
+    float2* y = reinterpret_cast<float2*>(scratch);
+    const size_t lds1 = sizeof(float2) * (size_t)P * pl.cols1 * (pl.N2 + 1);
+    const size_t lds2 = sizeof(float2) * (size_t)P * pl.cols2 * (pl.N1 + 1);
+    const void* f1 = reinterpret_cast<const void*>(&chan_big_pass1<P>);
+    const void* f2 = reinterpret_cast<const void*>(&chan_big_pass2<P>);
+    if (hipFuncSetAttribute(f1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1) != hipSuccess ||
+        hipFuncSetAttribute(f2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2) != hipSuccess)
+        return fail("channelizer: LDS attribute of the two-pass FFT");
+    const uint64_t t1 = n_spec * (uint64_t)(pl.N1 / pl.cols1), t2 = n_spec * (uint64_t)(pl.N2 / pl.cols2);
+    const unsigned g1 = (unsigned)std::min<uint64_t>(t1, 256 * 64), g2 = (unsigned)std::min<uint64_t>(t2, 256 * 64);
+    hipLaunchKernelGGL((chan_big_pass1<P>), dim3(g1), dim3(1024), lds1, s, in, y, C, pl.N1, pl.N2, pl.cols1,
+                       n_spec, tw2, twC, pl.r2);
+    rc = check_launch("channelizer pass 1");
+    if (rc) return rc;
+    hipLaunchKernelGGL((chan_big_pass2<P>), dim3(g2), dim3(1024), lds2, s, y, out, C, pl.N1, pl.N2, pl.cols2,
+                       n_spec, S, tw1, pl.r1);
+    return check_launch("channelizer pass 2");
+}
+
 template <int C, int P>
 static void launch_small(const float2* in, float2* out, uint64_t n_spec, uint64_t S, hipStream_t s) {
     SmallTw<C> tw;
@@ -1292,17 +1508,23 @@ extern "C" int blade_channelizer(const void* input, void* output, uint64_t A, ui
         }
         return BLADE_SUCCESS;
     }
-    if (input == output && S != 1)
-        return fail("channelizer: in-place operation needs rate == number of time samples");
     const float2* in = static_cast<const float2*>(input);
     float2* out = static_cast<float2*>(output);
+    if ((uint64_t)C * P > (uint64_t)kMaxLdsComplex) {
+        // Past one workgroup's LDS: two passes through a scratch buffer (which
+        // also makes any in-place call safe).
+        BigPlan pl;
+        if (!big_plan(C, (int)P, &pl))
+            return fail("channelizer: rate must be even, at most 2^20, and factor into 2, 3, 5 and 7");
+        return P == 2 ? launch_big<2>(pl, C, in, out, S, n_spec, s) : launch_big<1>(pl, C, in, out, S, n_spec, s);
+    }
+    if (input == output && S != 1)
+        return fail("channelizer: in-place operation needs rate == number of time samples");
     if (C <= 16 && input != output) {
         const int ok = (P == 2) ? dispatch_small<2>(C, in, out, n_spec, S, s)
                                 : dispatch_small<1>(C, in, out, n_spec, S, s);
         if (ok) return check_launch("channelizer");
     }
-    if ((uint64_t)C * P > (uint64_t)kMaxLdsComplex)
-        return fail("channelizer: rate * polarizations above 16384 is not supported");
     const float2* tw = nullptr;
     int rc = get_twiddles(C, s, &tw);
     if (rc) return rc;
@@ -1352,15 +1574,24 @@ extern "C" int blade_channelizer(const void* input, void* output, uint64_t A, ui
 
 extern "C" int blade_channelizer_plan(uint64_t rate, blade_stream_t stream) {
     if (rate == 0 || (rate % 2 && rate != 1)) return fail("channelizer: invalid rate");
-    if (rate > (uint64_t)kMaxLdsComplex) return fail("channelizer: rate above 16384 is not supported");
+    if (rate > (uint64_t)kMaxBigRate) return fail("channelizer: rate above 2^20 is not supported");
     if (rate <= 16) return BLADE_SUCCESS;   // register kernels carry their twiddles as arguments
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const float2* tw = nullptr;
-    return get_twiddles((int)rate, reinterpret_cast<hipStream_t>(stream), &tw);
+    int rc = get_twiddles((int)rate, s, &tw);
+    if (rc) return rc;
+    BigPlan pl;   // the two-pass FFT's tables (for two polarizations)
+    if (rate * 2 > (uint64_t)kMaxLdsComplex && big_plan((int)rate, 2, &pl)) {
+        rc = get_twiddles(pl.N1, s, &tw);
+        if (!rc) rc = get_twiddles(pl.N2, s, &tw);
+    }
+    return rc;
 }
 
 extern "C" int blade_channelizer_detector(const void* input, void* output, uint64_t A, uint64_t F,
                                           uint64_t T, uint64_t rate, uint64_t integration,
-                                          uint64_t out_pols, int accumulate, blade_stream_t stream) {
+                                          uint64_t out_pols, int polarize, int accumulate,
+                                          blade_stream_t stream) {
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     if (rate == 0 || T % rate) return fail("channelizer-detector: T must be divisible by the rate");
     if (rate % 2 && rate != 1) return fail("channelizer-detector: the rate should be divisible by 2");
@@ -1374,6 +1605,8 @@ extern "C" int blade_channelizer_detector(const void* input, void* output, uint6
     if (!input || !output) return fail("channelizer-detector: null pointer");
     if ((uintptr_t)input % 16 || (uintptr_t)output % 16) return fail("channelizer-detector: misaligned buffer");
     const int C = (int)rate;
+    // bit 0: add into the output (stepped integration); bit 1: XY -> LR first
+    const int flags = (accumulate ? 1 : 0) | (polarize ? 2 : 0);
     Radices rad;
     if (!factorize(C, &rad)) return fail("channelizer-detector: rate must factor into 2, 3, 5 and 7");
     const float2* tw = nullptr;
@@ -1382,17 +1615,17 @@ extern "C" int blade_channelizer_detector(const void* input, void* output, uint6
     if (use_fft16(C)) {
         int rc16 = 0;
         const float2* in16 = static_cast<const float2*>(input);
-        if (out_pols == 1 ? launch_fft16<1>(C, in16, output, S, A * F, (int)integration, accumulate, tw, s, &rc16)
-                          : launch_fft16<4>(C, in16, output, S, A * F, (int)integration, accumulate, tw, s, &rc16))
+        if (out_pols == 1 ? launch_fft16<1>(C, in16, output, S, A * F, (int)integration, flags, tw, s, &rc16)
+                          : launch_fft16<4>(C, in16, output, S, A * F, (int)integration, flags, tw, s, &rc16))
             return rc16;
     }
     Fft2Plan pl;
     if (fft2_plan(C, &pl)) {
         if (out_pols == 1)
             return launch_fft2<1>(pl, static_cast<const float2*>(input), output, C, S, A * F, (int)integration,
-                                  accumulate, tw, s);
+                                  flags, tw, s);
         return launch_fft2<4>(pl, static_cast<const float2*>(input), output, C, S, A * F, (int)integration,
-                              accumulate, tw, s);
+                              flags, tw, s);
     }
     int threads = 64;
     while (threads < 1024 && threads * 8 < 2 * C) threads *= 2;
@@ -1413,9 +1646,9 @@ extern "C" int blade_channelizer_detector(const void* input, void* output, uint6
     float* out = static_cast<float*>(output);
     if (out_pols == 1)
         hipLaunchKernelGGL((chan_detect_kernel<1, 8>), dim3((unsigned)blocks), dim3(threads), lds, s, in, out, C,
-                           (int)S, (int)integration, n_af, tw, rad, accumulate);
+                           (int)S, (int)integration, n_af, tw, rad, flags);
     else
         hipLaunchKernelGGL((chan_detect_kernel<4, 8>), dim3((unsigned)blocks), dim3(threads), lds, s, in, out, C,
-                           (int)S, (int)integration, n_af, tw, rad, accumulate);
+                           (int)S, (int)integration, n_af, tw, rad, flags);
     return check_launch("channelizer-detector");
 }

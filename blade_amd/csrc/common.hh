@@ -54,7 +54,7 @@ inline int device_cus() {
 // Library-owned scratch for launches that need it, one buffer per (device,
 // stream, consumer) so that calls on different streams never share one;
 // grown on demand (the old buffer is released after the stream drains).
-enum WorkspaceUser { WS_BEAMFORMER_SINK = 0, WS_ADJOINT = 1 };
+enum WorkspaceUser { WS_BEAMFORMER_SINK = 0, WS_ADJOINT = 1, WS_CHANNELIZER_BIG = 2 };
 int library_workspace(int user, size_t bytes, hipStream_t s, unsigned char** out);
 
 // Number of scalar components and bytes per element of a blade_type.

@@ -10,6 +10,7 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <cstring>
 #include <sstream>
 
 #include "blade/base.hh"
@@ -19,6 +20,16 @@ using namespace pybind11::literals;
 using namespace Blade;
 
 namespace {
+
+// Pinned or pageable host memory (not device or managed memory).
+bool is_host_memory(const void* p) {
+    hipPointerAttribute_t attr{};
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+        (void)hipGetLastError();   // unknown to HIP: pageable host memory
+        return true;
+    }
+    return !attr.isManaged && (attr.type == hipMemoryTypeHost || attr.type == hipMemoryTypeUnregistered);
+}
 
 // Result codes surface as exceptions only from constructors (like the
 // reference, python/blade/ext_hidden.cc:17-24).
@@ -601,6 +612,12 @@ PYBIND11_MODULE(_blade_ext, m) {
     cp.def("copy_bytes", [](uintptr_t dst, uintptr_t src, U64 nbytes, uintptr_t stream, bool sync) {
         if (nbytes == 0 || dst == src) return Result::SUCCESS;
         hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+        // Host to host: finished on return, like Copy (include/blade/memory/copy.hh).
+        if (is_host_memory(reinterpret_cast<const void*>(dst)) && is_host_memory(reinterpret_cast<const void*>(src))) {
+            if (hipStreamSynchronize(s) != hipSuccess) return Result::CUDA_ERROR;
+            std::memcpy(reinterpret_cast<void*>(dst), reinterpret_cast<const void*>(src), nbytes);
+            return Result::SUCCESS;
+        }
         hipError_t e = hipMemcpyAsync(reinterpret_cast<void*>(dst), reinterpret_cast<const void*>(src), nbytes,
                                       hipMemcpyDefault, s);
         if (e == hipSuccess && sync) e = hipStreamSynchronize(s);

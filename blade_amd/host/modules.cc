@@ -59,7 +59,17 @@ Result Generic<IT, OT>::process(const U64&, const Stream& stream) {
         TypeInfo<IT>::abi, prec, input.buf.data(), input.phasors.data(), output.buf.data(),
         s.numberOfAspects(), s.numberOfFrequencyChannels(), s.numberOfTimeSamples(),
         s.numberOfPolarizations(), B, config.enableIncoherentBeam, config.enableIncoherentBeamSqrt,
-        1, 0, workspace.data(), workspace.size(), abi(stream)));
+        epilogueRate, epiloguePolarize ? 1 : 0, workspace.data(), workspace.size(), abi(stream)));
+    return Result::SUCCESS;
+}
+
+template <typename IT, typename OT>
+Result Generic<IT, OT>::fuseEpilogue(U64 rate, bool polarize) {
+    const U64 T = getInputBuffer().shape().numberOfTimeSamples();
+    if (rate != 1 && rate != 2 && rate != 4) return Result::ERROR;
+    if (T % rate != 0 || (rate > 1 && T % 4 != 0)) return Result::ERROR;
+    epilogueRate = rate;
+    epiloguePolarize = polarize;
     return Result::SUCCESS;
 }
 
@@ -165,7 +175,7 @@ namespace Modules {
 
 template <typename IT, typename OT>
 Channelizer<IT, OT>::Channelizer(const Config& config, const Input& input, const Stream& stream)
-    : config(config), input(input), inPlace(false) {
+    : config(config), input(input), isFused(false) {
     const auto& s = getInputBuffer().shape();
     if (config.rate == 0 || s.numberOfTimeSamples() % config.rate != 0) {
         BL_FATAL("The number of time samples ({}) should be divisable by the channelizer rate ({}).",
@@ -181,12 +191,13 @@ Channelizer<IT, OT>::Channelizer(const Config& config, const Input& input, const
                  s.numberOfPolarizations());
         throw Result::ERROR;
     }
-    // rate 1 and rate T keep the memory layout, so they run in place.
-    if (config.rate == 1 || config.rate == s.numberOfTimeSamples()) {
-        inPlace = true;
-        BL_CHECK_THROW(Link(output.buf, input.buf, getOutputBufferShape()));
-    } else {
-        output.buf = ArrayTensor<Device::CUDA, OT>(getOutputBufferShape());
+    // In place for every rate (reference base.cc:45).  The spectra of one
+    // (aspect, channel) occupy the same T*P samples as its input, but for
+    // 1 < rate < T in another order, so the transform writes a staging
+    // buffer first.
+    BL_CHECK_THROW(Link(output.buf, input.buf, getOutputBufferShape()));
+    if (config.rate != 1 && config.rate != s.numberOfTimeSamples()) {
+        staging = ArrayTensor<Device::CUDA, OT>(getOutputBufferShape());
     }
     if (config.rate > 1 && !Profiler::IsCapturing()) {
         if (blade_channelizer_plan(config.rate, stream.abi()) != 0) {
@@ -201,15 +212,109 @@ Channelizer<IT, OT>::Channelizer(const Config& config, const Input& input, const
 
 template <typename IT, typename OT>
 Result Channelizer<IT, OT>::process(const U64&, const Stream& stream) {
-    if (config.rate == 1) return Result::SUCCESS;
+    // Follow the input handle: a Runner re-points a Duet to the block's slot
+    // before each cycle (the reference links once, in the constructor).
+    if (output.buf.data() != input.buf.data()) BL_CHECK(Link(output.buf, input.buf, getOutputBufferShape()));
+    if (config.rate == 1 || isFused) return Result::SUCCESS;
     const auto& s = getInputBuffer().shape();
-    BL_ABI_CHECK(blade_channelizer(input.buf.data(), output.buf.data(), s.numberOfAspects(),
+    if (staging.empty()) {
+        BL_ABI_CHECK(blade_channelizer(input.buf.data(), output.buf.data(), s.numberOfAspects(),
+                                       s.numberOfFrequencyChannels(), s.numberOfTimeSamples(),
+                                       s.numberOfPolarizations(), config.rate, abi(stream)));
+        return Result::SUCCESS;
+    }
+    BL_ABI_CHECK(blade_channelizer(input.buf.data(), staging.data(), s.numberOfAspects(),
                                    s.numberOfFrequencyChannels(), s.numberOfTimeSamples(),
                                    s.numberOfPolarizations(), config.rate, abi(stream)));
+    BL_ABI_CHECK(blade_gather(staging.data(), output.buf.data(), 1, staging.size_bytes(), 1, 0,
+                              abi(stream)));
     return Result::SUCCESS;
 }
 
+template <typename IT, typename OT>
+void Channelizer<IT, OT>::fuseIntoProducer() {
+    isFused = true;
+    staging = ArrayTensor<Device::CUDA, OT>();   // not needed any more
+}
+
 template class BLADE_API Channelizer<CF32, CF32>;
+
+// ------------------------------------------------------- ChannelizerDetector
+
+#undef BL_LOG_DOMAIN
+#define BL_LOG_DOMAIN "M::CHANNELIZER_DETECTOR"
+
+template <typename IT, typename OT>
+bool ChannelizerDetector<IT, OT>::Supports(U64 rate) {
+    if (rate == 0 || rate > 8192) return false;
+    U64 r = rate;
+    for (U64 p : {2, 3, 5, 7}) {
+        while (r % p == 0) r /= p;
+    }
+    return r == 1;
+}
+
+template <typename IT, typename OT>
+ChannelizerDetector<IT, OT>::ChannelizerDetector(const Config& config, const Input& input, const Stream& stream)
+    : config(config), input(input), steps(1) {
+    const auto& s = getInputBuffer().shape();
+    if (config.rate == 0 || s.numberOfTimeSamples() % config.rate != 0) {
+        BL_FATAL("The number of time samples ({}) should be divisable by the channelizer rate ({}).",
+                 s.numberOfTimeSamples(), config.rate);
+        throw Result::ERROR;
+    }
+    if ((config.rate % 2) != 0 && config.rate != 1) {
+        BL_FATAL("The channelizer rate ({}) should be divisable by 2.", config.rate);
+        throw Result::ERROR;
+    }
+    if (!Supports(config.rate)) {
+        BL_FATAL("The fused channelizer-detector does not run rate {}.", config.rate);
+        throw Result::ERROR;
+    }
+    if (s.numberOfPolarizations() != 2) {
+        BL_FATAL("Number of polarizations ({}) should be two (2).", s.numberOfPolarizations());
+        throw Result::ERROR;
+    }
+    if (config.integrationSize == 0) {
+        BL_WARN("Integration size ({}) should be more than zero.", config.integrationSize);
+        throw Result::ERROR;
+    }
+    const U64 S = spectra();
+    if (config.integrationSize > S && config.integrationSize % S == 0) {
+        steps = config.integrationSize / S;
+    } else if (S % config.integrationSize != 0) {
+        BL_FATAL("The number of spectra ({}) should be divisable by the integration size ({}).", S,
+                 config.integrationSize);
+        throw Result::ERROR;
+    }
+    if (config.numberOfOutputPolarizations != 4 && config.numberOfOutputPolarizations != 1) {
+        BL_FATAL("Number of output polarizations ({}) not supported.", config.numberOfOutputPolarizations);
+        throw Result::ERROR;
+    }
+    output.buf = ArrayTensor<Device::CUDA, OT>(getOutputBufferShape());
+    if (!Profiler::IsCapturing() && blade_channelizer_plan(config.rate, stream.abi()) != 0) {
+        BL_FATAL("{}", blade_amd_last_error());
+        throw Result::ERROR;
+    }
+    BL_INFO("Type: {} -> {}", TypeInfo<IT>::name, TypeInfo<OT>::name);
+    BL_INFO("Shape: {} -> {}", s, getOutputBuffer().shape());
+    BL_INFO("FFT Size: {}, Circular: {}, Integration Size: {} ({} block(s))", config.rate,
+            config.convertToCircular ? "YES" : "NO", config.integrationSize, steps);
+}
+
+template <typename IT, typename OT>
+Result ChannelizerDetector<IT, OT>::process(const U64& currentStepCount, const Stream& stream) {
+    const auto& s = getInputBuffer().shape();
+    const int accumulate = (steps > 1 && currentStepCount > 0) ? 1 : 0;
+    BL_ABI_CHECK(blade_channelizer_detector(input.buf.data(), output.buf.data(), s.numberOfAspects(),
+                                            s.numberOfFrequencyChannels(), s.numberOfTimeSamples(),
+                                            config.rate, config.integrationSize,
+                                            config.numberOfOutputPolarizations,
+                                            config.convertToCircular ? 1 : 0, accumulate, abi(stream)));
+    return Result::SUCCESS;
+}
+
+template class BLADE_API ChannelizerDetector<CF32, F32>;
 
 // ----------------------------------------------------------------- Polarizer
 // (reference src/modules/polarizer/base.cc:12-71)
@@ -234,7 +339,10 @@ Polarizer<IT, OT>::Polarizer(const Config& config, const Input& input, const Str
 
 template <typename IT, typename OT>
 Result Polarizer<IT, OT>::process(const U64&, const Stream& stream) {
-    if (config.inputPolarization == config.outputPolarization) return Result::SUCCESS;
+    if constexpr (std::is_same<IT, OT>::value) {
+        if (output.buf.data() != input.buf.data()) BL_CHECK(Link(output.buf, input.buf));   // see Channelizer
+    }
+    if (config.inputPolarization == config.outputPolarization || isFused) return Result::SUCCESS;
     BL_ABI_CHECK(blade_polarizer(TypeInfo<IT>::abi, input.buf.data(), output.buf.data(),
                                  input.buf.size(), abi(stream)));
     return Result::SUCCESS;
@@ -324,6 +432,7 @@ Cast<IT, OT>::Cast(const Config& config, const Input& input, const Stream&)
 template <typename IT, typename OT>
 Result Cast<IT, OT>::process(const U64&, const Stream& stream) {
     if constexpr (std::is_same<IT, OT>::value) {
+        if (output.buf.data() != input.buf.data()) BL_CHECK(Link(output.buf, input.buf));   // see Channelizer
         return Result::SUCCESS;
     } else {
         BL_ABI_CHECK(blade_cast(TypeInfo<IT>::abi, TypeInfo<OT>::abi, input.buf.data(),

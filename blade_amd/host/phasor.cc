@@ -18,6 +18,11 @@
 #include "blade/base.hh"
 
 #include <cmath>
+#include <string>
+
+namespace bl {
+void set_error(const std::string& msg);   // blade_amd_last_error() message (runtime.cc)
+}
 
 namespace Blade::Modules::Phasor {
 
@@ -98,7 +103,52 @@ void precess(F64 jd, F64& ra, F64& dec) {
     dec = std::asin(std::max(-1.0, std::min(1.0, C)));
 }
 
+// w of every antenna relative to the reference antenna, divided by c, for a
+// J2000 (ra, dec) at a UTC Julian date: the delay (s) of each antenna's
+// signal after the reference antenna's.
+void position_delays(const std::vector<XYZ>& xyz, U64 ref, F64 lon, F64 ra, F64 dec, F64 jd,
+                     F64 dut1, F64* delays) {
+    const F64 jd_ut1 = jd + dut1 / 86400.0;
+    precess(jd, ra, dec);
+    const F64 H = gmst(jd_ut1) + lon - ra;
+    const F64 cd = std::cos(dec), sd = std::sin(dec), ch = std::cos(H), sh = std::sin(H);
+    const XYZ& r = xyz[ref];
+    const F64 wr = cd * ch * r.X - cd * sh * r.Y + sd * r.Z;
+    for (U64 a = 0; a < xyz.size(); a++) {
+        const XYZ& p = xyz[a];
+        const F64 w = cd * ch * p.X - cd * sh * p.Y + sd * p.Z;
+        delays[a] = (w - wr) / BL_PHYSICAL_CONSTANT_C;
+    }
+}
+
+// ECEF antenna positions -> baselines from the array reference position in
+// the local XYZ frame (rotation by the reference longitude).
+std::vector<XYZ> local_xyz(const XYZ* ecef, U64 A, const LLA& refPos) {
+    const XYZ ref = lla_to_ecef(refPos);
+    const F64 cl = std::cos(refPos.LON), sl = std::sin(refPos.LON);
+    std::vector<XYZ> xyz(A);
+    for (U64 i = 0; i < A; i++) {
+        const XYZ& p = ecef[i];
+        const F64 bx = p.X - ref.X, by = p.Y - ref.Y, bz = p.Z - ref.Z;
+        xyz[i] = {bx * cl + by * sl, -bx * sl + by * cl, bz};
+    }
+    return xyz;
+}
+
 }  // namespace
+
+// Beam-minus-boresight delays [B][A] (reference ata.cc:105-173).
+void ataDelays(const std::vector<XYZ>& xyz, U64 ref, F64 lon, const RA_DEC& boresight,
+               const RA_DEC* beams, U64 B, F64 jd, F64 dut1, F64* delays) {
+    const U64 A = xyz.size();
+    std::vector<F64> bore(A);
+    position_delays(xyz, ref, lon, boresight.RA, boresight.DEC, jd, dut1, bore.data());
+    for (U64 b = 0; b < B; b++) {
+        F64* d = delays + b * A;
+        position_delays(xyz, ref, lon, beams[b].RA, beams[b].DEC, jd, dut1, d);
+        for (U64 a = 0; a < A; a++) d[a] -= bore[a];
+    }
+}
 
 template <typename OT>
 ATA<OT>::ATA(const typename Generic<OT>::Config& config, const typename Generic<OT>::Input& input,
@@ -117,15 +167,8 @@ ATA<OT>::ATA(const typename Generic<OT>::Config& config, const typename Generic<
         throw Result::ERROR;
     }
     // ECEF -> array-centred XYZ (rotation by the reference longitude).
-    const XYZ ref = lla_to_ecef(config.arrayReferencePosition);
-    const F64 cl = std::cos(config.arrayReferencePosition.LON);
-    const F64 sl = std::sin(config.arrayReferencePosition.LON);
-    antennasXyz.resize(config.numberOfAntennas);
-    for (U64 i = 0; i < config.numberOfAntennas; i++) {
-        const XYZ& p = config.antennaPositions[i];
-        const F64 bx = p.X - ref.X, by = p.Y - ref.Y, bz = p.Z - ref.Z;
-        antennasXyz[i] = {bx * cl + by * sl, -bx * sl + by * cl, bz};
-    }
+    antennasXyz = local_xyz(config.antennaPositions.data(), config.numberOfAntennas,
+                            config.arrayReferencePosition);
     this->output.phasors = PhasorTensor<Device::CUDA, OT>(getOutputPhasorsShape());
     this->output.delays = DelayTensor<Device::CPU, F64>(getOutputDelaysShape());
     deviceDelays = DelayTensor<Device::CUDA, F64>(getOutputDelaysShape());
@@ -145,22 +188,6 @@ ATA<OT>::~ATA() {
 }
 
 template <typename OT>
-void ATA<OT>::positionDelays(F64 ra, F64 dec, F64 jd, F64 dut1, F64* delays) const {
-    const F64 jd_ut1 = jd + dut1 / 86400.0;
-    precess(jd, ra, dec);
-    const F64 H = gmst(jd_ut1) + this->config.arrayReferencePosition.LON - ra;
-    const F64 cd = std::cos(dec), sd = std::sin(dec), ch = std::cos(H), sh = std::sin(H);
-    const U64 A = this->config.numberOfAntennas;
-    const XYZ& r = antennasXyz[this->config.referenceAntennaIndex];
-    const F64 wr = cd * ch * r.X - cd * sh * r.Y + sd * r.Z;
-    for (U64 a = 0; a < A; a++) {
-        const XYZ& p = antennasXyz[a];
-        const F64 w = cd * ch * p.X - cd * sh * p.Y + sd * p.Z;
-        delays[a] = (w - wr) / BL_PHYSICAL_CONSTANT_C;
-    }
-}
-
-template <typename OT>
 Result ATA<OT>::process(const U64&, const Stream& stream) {
     const auto& cfg = this->config;
     const U64 A = cfg.numberOfAntennas, B = cfg.beamCoordinates.size();
@@ -169,13 +196,9 @@ Result ATA<OT>::process(const U64&, const Stream& stream) {
     // The previous block's upload (possibly on the other stream) must have
     // read the pinned host delays before they are overwritten.
     if (uploaded) BL_HIP_CHECK(hipEventSynchronize(uploaded));
-    std::vector<F64> bore(A);
-    positionDelays(cfg.boresightCoordinate.RA, cfg.boresightCoordinate.DEC, jd, dut1, bore.data());
-    for (U64 b = 0; b < B; b++) {
-        F64* d = this->output.delays.data() + b * A;
-        positionDelays(cfg.beamCoordinates[b].RA, cfg.beamCoordinates[b].DEC, jd, dut1, d);
-        for (U64 a = 0; a < A; a++) d[a] -= bore[a];
-    }
+    ataDelays(antennasXyz, cfg.referenceAntennaIndex, cfg.arrayReferencePosition.LON,
+              cfg.boresightCoordinate, cfg.beamCoordinates.data(), B, jd, dut1,
+              this->output.delays.data());
     BL_CHECK(Copy(deviceDelays, this->output.delays, stream));
     if (uploaded) BL_HIP_CHECK(hipEventRecord(uploaded, stream.handle));
     BL_ABI_CHECK(blade_phasor_ata(deviceDelays.data(), deviceCalibrations.data(),
@@ -193,3 +216,26 @@ template class BLADE_API ATA<CF32>;
 template class BLADE_API ATA<CF64>;
 
 }  // namespace Blade::Modules::Phasor
+
+extern "C" int blade_ata_delays(const double* antenna_ecef, uint64_t A, uint64_t reference_antenna,
+                                const double* reference_lla, const double* boresight_radec,
+                                const double* beam_radec, uint64_t B, double julian_date,
+                                double dut1, double* delays) {
+    using namespace Blade;
+    using namespace Blade::Modules::Phasor;
+    if (A == 0 || B == 0) return BLADE_SUCCESS;
+    if (!antenna_ecef || !reference_lla || !boresight_radec || !beam_radec || !delays) {
+        bl::set_error("ata delays: null pointer");
+        return BLADE_ERROR;
+    }
+    if (reference_antenna >= A) {
+        bl::set_error("ata delays: reference antenna index is not below the number of antennas");
+        return BLADE_ERROR;
+    }
+    const LLA ref{reference_lla[0], reference_lla[1], reference_lla[2]};
+    const std::vector<XYZ> xyz = local_xyz(reinterpret_cast<const XYZ*>(antenna_ecef), A, ref);
+    const RA_DEC bore{boresight_radec[0], boresight_radec[1]};
+    ataDelays(xyz, reference_antenna, ref.LON, bore, reinterpret_cast<const RA_DEC*>(beam_radec), B,
+              julian_date, dut1, delays);
+    return BLADE_SUCCESS;
+}

@@ -16,13 +16,14 @@ Profiler& Profiler::Get() {
 
 Pipeline::Pipeline()
     : _commited(false), _graphs(false), _computeStepCount(0), _computeStepsPerCycle(1),
-      _computeLifetimeCycles(0) {
+      _computeLifetimeCycles(0), _cycleDone(nullptr), _cycleRecorded(false) {
     _streams.resize(2);
     _graphExec.assign(2, nullptr);
     for (auto& s : _streams) {
         // Default priority; non-blocking with respect to the null stream.
         BL_HIP_CHECK_THROW(hipStreamCreateWithFlags(&s.handle, hipStreamNonBlocking));
     }
+    BL_HIP_CHECK_THROW(hipEventCreateWithFlags(&_cycleDone, hipEventDisableTiming));
 }
 
 Pipeline::~Pipeline() {
@@ -33,7 +34,14 @@ Pipeline::~Pipeline() {
         if (g) (void)hipGraphExecDestroy(g);
     }
     _modules.clear();
+    if (_cycleDone) (void)hipEventDestroy(_cycleDone);
     for (auto& s : _streams) (void)hipStreamDestroy(s.handle);
+}
+
+Result Pipeline::markCycleDone(const U64& index) {
+    BL_HIP_CHECK(hipEventRecord(_cycleDone, _streams[index].handle));
+    _cycleRecorded = true;
+    return Result::SUCCESS;
 }
 
 void Pipeline::addModule(const std::shared_ptr<Module>& module) {
@@ -62,18 +70,80 @@ bool Pipeline::isSynchronized(const U64& index) {
     return hipStreamQuery(_streams[index].handle) == hipSuccess;
 }
 
+namespace {
+
+// Beamformer -> [Channelizer of rate 1, 2 or 4] -> [Polarizer], each reading
+// and writing the beamformer's output buffer in place and adjacent in the
+// module order, become one launch of the beamformer with a fused epilogue.
+// The buffer ends up as after the three modules ran one by one (the
+// channelizer and polarizer alias it like the reference's Link,
+// src/modules/channelizer/base.cc:45, polarizer/base.cc:54), so the fusion
+// is invisible except in the launch count.
+template <typename IT>
+bool fuseBeamformerChain(const std::vector<std::shared_ptr<Module>>& mods, U64 i) {
+    auto* bf = dynamic_cast<Modules::Beamformer::Generic<IT, CF32>*>(mods[i].get());
+    if (!bf) return false;
+    const void* buf = bf->getOutputBuffer().data();
+    U64 j = i + 1;
+    U64 rate = 1;
+    Modules::Channelizer<CF32, CF32>* ch = nullptr;
+    Modules::Polarizer<CF32, CF32>* po = nullptr;
+    if (j < mods.size()) {
+        ch = dynamic_cast<Modules::Channelizer<CF32, CF32>*>(mods[j].get());
+        if (ch && ch->getInputBuffer().data() == buf && ch->getOutputBuffer().data() == buf &&
+            ch->getConfig().rate <= 4 && ch->getConfig().rate != 3) {
+            rate = ch->getConfig().rate;
+            j += 1;
+        } else {
+            ch = nullptr;
+        }
+    }
+    if (j < mods.size()) {
+        po = dynamic_cast<Modules::Polarizer<CF32, CF32>*>(mods[j].get());
+        if (!(po && po->getInputBuffer().data() == buf && po->getOutputBuffer().data() == buf &&
+              po->getConfig().inputPolarization != po->getConfig().outputPolarization)) {
+            po = nullptr;
+        }
+    }
+    if (rate == 1 && !po) return false;       // nothing to fuse
+    if (bf->fuseEpilogue(rate, po != nullptr) != Result::SUCCESS) return false;
+    if (ch) ch->fuseIntoProducer();
+    if (po) po->fuseIntoProducer();
+    BL_INFO("Fused Beamformer{}{} into one launch.", ch ? " -> Channelizer" : "",
+            po ? " -> Polarizer" : "");
+    return true;
+}
+
+}  // namespace
+
 Result Pipeline::commit() {
     BL_DEBUG("Commiting pipeline with {} modules and {} compute steps per cycle.", _modules.size(),
              _computeStepsPerCycle);
+    for (U64 i = 0; i < _modules.size(); i++) {
+        if (!fuseBeamformerChain<CI8>(_modules, i)) (void)fuseBeamformerChain<CF32>(_modules, i);
+    }
     _computeStepRatios.push_back(1);
     return Result::SUCCESS;
 }
 
-Result Pipeline::enableGraphCapture(bool enable) {
-    if (enable && _computeStepsPerCycle != 1) {
+// A captured graph replays the device work of one call: every module must
+// run every cycle (no CHRONOUS step ratios) and do no host work in process().
+Result Pipeline::checkCapturable() const {
+    if (_computeStepsPerCycle != 1) {
         BL_FATAL("Graph capture needs a pipeline without CHRONOUS modules.");
         return Result::ERROR;
     }
+    for (const auto& module : _modules) {
+        if (!module->graphCapturable()) {
+            BL_FATAL("Graph capture: module '{}' does host work in process().", module->name());
+            return Result::ERROR;
+        }
+    }
+    return Result::SUCCESS;
+}
+
+Result Pipeline::enableGraphCapture(bool enable) {
+    if (enable) BL_CHECK(checkCapturable());
     _graphs = enable;
     return Result::SUCCESS;
 }
@@ -113,9 +183,14 @@ Result Pipeline::compute(const U64& index) {
         BL_CHECK(commit());
         _commited = true;
     }
+    // The previous cycle (on either stream) must be over before this one
+    // writes the modules' shared outputs.
+    if (_cycleRecorded) BL_HIP_CHECK(hipStreamWaitEvent(_streams[index].handle, _cycleDone, 0));
     if (_graphs) {
         hipStream_t s = _streams[index].handle;
         if (!_graphExec[index]) {
+            // Modules connected after enableGraphCapture() are checked here.
+            BL_CHECK(checkCapturable());
             hipGraph_t graph = nullptr;
             BL_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             const Result r = runModules(index);
@@ -138,6 +213,7 @@ Result Pipeline::compute(const U64& index) {
         BL_FATAL("HIP compute error: {}", hipGetErrorString(e));
         return Result::CUDA_ERROR;
     }
+    BL_CHECK(markCycleDone(index));
     _computeStepCount += 1;
     if (_computeStepCount == _computeStepsPerCycle) {
         _computeStepCount = 0;
@@ -166,7 +242,11 @@ Result Runner::enqueue(const std::function<Result()>& inputCallback,
              outputId, out ? "Y" : "N");
     BL_CHECK(inputCallback());
     BL_CHECK(compute(headIndex));
-    if (out) BL_CHECK(outputCallback());
+    if (out) {
+        BL_CHECK(outputCallback());
+        // The next cycle's compute also waits for these output copies.
+        BL_CHECK(markCycleDone(headIndex));
+    }
     queue.push({headIndex, inputId, outputId, out});
     headIndex = (headIndex + 1) % numberOfStreams();
     return Result::SUCCESS;

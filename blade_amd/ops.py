@@ -208,15 +208,17 @@ def channelizer(x: torch.Tensor, rate: int, out: torch.Tensor | None = None, str
 
 
 def channelizer_detector(x: torch.Tensor, rate: int, integration: int, out_pols: int = 1,
-                         out: torch.Tensor | None = None, accumulate=False, stream=None):
-    """Fused channelize -> detect, CF32 [A,F,T,2] -> F32 [A,F*C,S/N or 1,out_pols]."""
+                         out: torch.Tensor | None = None, accumulate=False, stream=None,
+                         polarize=False):
+    """Fused channelize -> [polarize XY -> LR] -> detect,
+    CF32 [A,F,T,2] -> F32 [A,F*C,S/N or 1,out_pols]."""
     A, F, T, P = _shape4(x, "cf32")
     S = T // rate
     groups = S // integration if integration <= S else 1
     if out is None:
         out = torch.empty((A, F * rate, groups, out_pols), dtype=torch.float32, device=x.device)
     call("blade_channelizer_detector", _ptr(x), _ptr(out), A, F, T, rate, integration, out_pols,
-         int(accumulate), stream_handle(stream))
+         int(polarize), int(accumulate), stream_handle(stream))
     return out
 
 

@@ -2,10 +2,19 @@
 // The HIP runtime infers the direction from the pointers (hipMemcpyDefault),
 // so every Copy/Copy2D overload uses the right kind (the reference passes
 // DeviceToHost for every 2-D copy, copy.hh:151-180).
+//
+// Host-to-host copies have finished when Copy returns, as CUDA's
+// cudaMemcpyAsync does for them (the reference's CPU -> CPU Copy relies on
+// it: modules read host inputs such as the Julian date in process(), right
+// after the Runner's input callback copied them).  HIP would queue them on
+// the stream instead, so here the stream's earlier work is awaited and the
+// bytes are copied by the host.
 #ifndef BLADE_MEMORY_COPY_HH
 #define BLADE_MEMORY_COPY_HH
 
 #include <hip/hip_runtime_api.h>
+
+#include <cstring>
 
 #include "blade/macros.hh"
 #include "blade/memory/vector.hh"
@@ -24,6 +33,13 @@ inline Result Copy(Vector<D, Type, Dims>& dst, const Vector<S, Type, Dims>& src,
         return Result::ASSERTION_ERROR;
     }
     if (src.size_bytes() == 0 || dst.data() == src.data()) return Result::SUCCESS;
+    if constexpr (D == Device::CPU && S == Device::CPU) {
+        if (!dst.unified() && !src.unified()) {
+            BL_HIP_CHECK(hipStreamSynchronize(stream.handle));
+            std::memcpy(dst.data(), src.data(), src.size_bytes());
+            return Result::SUCCESS;
+        }
+    }
     BL_HIP_CHECK(hipMemcpyAsync(dst.data(), src.data(), src.size_bytes(), hipMemcpyDefault,
                                 stream.handle));
     return Result::SUCCESS;

@@ -5,6 +5,7 @@
 #include "blade/modules/beamformer/batched.hh"
 #include "blade/modules/cast.hh"
 #include "blade/modules/channelizer/base.hh"
+#include "blade/modules/channelizer/detector.hh"
 #include "blade/modules/detector.hh"
 #include "blade/modules/duplicate.hh"
 #include "blade/modules/gather.hh"

@@ -43,12 +43,23 @@ class BLADE_API Generic : public Module {
     virtual ~Generic() = default;
     Result process(const U64& currentStepCount, const Stream& stream = {}) override;
 
+    // Pipeline fusion (no reference counterpart): run the Channelizer of
+    // `rate` (1, 2 or 4) and, with `polarize`, the XY -> LR Polarizer that
+    // follow this module in place on its output buffer, inside this
+    // module's own launch.  The buffer then ends up as it would after the
+    // three modules ran one by one.
+    Result fuseEpilogue(U64 rate, bool polarize);
+    U64 fusedRate() const { return epilogueRate; }
+    bool fusedPolarize() const { return epiloguePolarize; }
+
  protected:
     const Config config;
     const Input input;
     Output output;
-    // Packed matrix-core operands of the phasors (rebuilt every process()).
+    // Sink for the stores of idle lanes (allocated once; see the C-ABI).
     Vector<Device::CUDA, U8, VectorShape> workspace;
+    U64 epilogueRate = 1;
+    bool epiloguePolarize = false;
 
     virtual const ArrayShape getOutputBufferShape() const = 0;
 };

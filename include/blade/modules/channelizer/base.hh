@@ -1,7 +1,9 @@
 // Channelizer module (reference include/blade/modules/channelizer/base.hh:17-86).
 // Any rate dividing T is accepted (the reference accepts only 1 and T,
-// src/modules/channelizer/base.cc:30-35).  The output is a separate tensor
-// unless rate == T, where the transform runs in place like the reference.
+// src/modules/channelizer/base.cc:30-35).  The output aliases the input
+// buffer for every rate, as the reference links them (base.cc:45): rate T
+// transforms in place, other rates go through a staging buffer of the
+// module.  A Pipeline fuses a rate of 2 or 4 into a Beamformer before it.
 #ifndef BLADE_MODULES_CHANNELIZER_BASE_HH
 #define BLADE_MODULES_CHANNELIZER_BASE_HH
 
@@ -39,11 +41,18 @@ class BLADE_API Channelizer : public Module {
     explicit Channelizer(const Config& config, const Input& input, const Stream& stream = {});
     Result process(const U64& currentStepCount, const Stream& stream = {}) override;
 
+    // Pipeline fusion (no reference counterpart): the producer of the input
+    // buffer runs this transform in its own launch; process() does nothing.
+    void fuseIntoProducer();
+    bool fused() const { return isFused; }
+
  private:
     const Config config;
     const Input input;
     Output output;
-    bool inPlace;
+    bool isFused;
+    // Out-of-place result for 1 < rate < T, copied back over the buffer.
+    ArrayTensor<Device::CUDA, OT> staging;
 
     const ArrayShape getOutputBufferShape() const {
         return ArrayShape({

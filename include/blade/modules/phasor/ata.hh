@@ -20,10 +20,6 @@ class BLADE_API ATA : public Generic<OT> {
     ~ATA();
     Result process(const U64& currentStepCount, const Stream& stream = {}) final;
 
-    // Geometric delays (seconds) of every antenna towards (ra, dec) at
-    // Julian date jd (UT1 = UTC + dut1), relative to the reference antenna.
-    void positionDelays(F64 ra, F64 dec, F64 jd, F64 dut1, F64* delays) const;
-
  protected:
     const PhasorShape getOutputPhasorsShape() const {
         return PhasorShape({this->config.beamCoordinates.size(), this->config.numberOfAntennas,

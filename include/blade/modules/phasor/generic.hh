@@ -53,6 +53,8 @@ class BLADE_API Generic : public Module {
 
     Taint getTaint() const override { return Taint::CONSUMER | Taint::PRODUCER; }
     std::string name() const override { return "Phasor"; }
+    // The delays are computed on the host for each block's Julian date.
+    bool graphCapturable() const override { return false; }
 
     explicit Generic(const Config& config, const Input& input, const Stream& stream = {});
     virtual ~Generic() = default;

@@ -38,10 +38,16 @@ class BLADE_API Polarizer : public Module {
     explicit Polarizer(const Config& config, const Input& input, const Stream& stream = {});
     Result process(const U64& currentStepCount, const Stream& stream = {}) override;
 
+    // Pipeline fusion (no reference counterpart): the producer of the input
+    // buffer polarizes in its own launch; process() does nothing.
+    void fuseIntoProducer() { isFused = true; }
+    bool fused() const { return isFused; }
+
  private:
     const Config config;
     const Input input;
     Output output;
+    bool isFused = false;
 };
 
 }  // namespace Blade::Modules

@@ -3,6 +3,13 @@
 // src/pipeline.cc:7-141).  Optionally each stream's module sequence is
 // captured once into a hipGraph and replayed (pipelines without CHRONOUS
 // modules and without host-side work in process()).
+//
+// Cross-stream order.  Every module owns one output buffer, shared by both
+// streams, and CHRONOUS modules (Gather, a stepped Detector) accumulate into
+// theirs across cycles.  So a cycle's compute waits on the device for the
+// previous cycle to end: its compute, and under a Runner its output copies.
+// The input copies of the next cycle (into the other slot of a Duet) are
+// not ordered and overlap the previous cycle's compute.
 #ifndef BLADE_PIPELINE_HH
 #define BLADE_PIPELINE_HH
 
@@ -70,8 +77,18 @@ class BLADE_API Pipeline {
     U64 _computeLifetimeCycles;
     std::vector<U64> _computeStepRatios;
 
+    // Recorded at the end of the last cycle (compute, or the Runner's output
+    // callback); the next compute on any stream waits on it.
+    hipEvent_t _cycleDone;
+    bool _cycleRecorded;
+
     Result commit();
     Result runModules(const U64& index);
+    Result checkCapturable() const;
+
+ protected:
+    // Mark the work issued so far on stream `index` as the end of the cycle.
+    Result markCycleDone(const U64& index);
 };
 
 }  // namespace Blade

@@ -175,8 +175,10 @@ int blade_beamformer_stokes_i_grad(const void* input, const void* phasors,
  * pre-kernel channelizer.cu:5-19 and cuFFT C2C forward) with the general-rate
  * semantics of the oracle tests/modules/channelizer/advanced.py:49-65:
  *   out[a, f*C+k, s, p] = fftshift(fft(in[a, f, s*C:(s+1)*C, p]))[k]
- * CF32 [A,F,T,P] -> CF32 [A,F*C,T/C,P]; rate 1 copies; C <= 16384.
- * in == out (in-place) is supported.
+ * CF32 [A,F,T,P] -> CF32 [A,F*C,T/C,P]; rate 1 copies; C up to 2^20 and,
+ * when C * P > 16384, factoring into 2, 3, 5 and 7 (two passes through a
+ * library-owned scratch buffer of the data's size).  in == out (in-place) is
+ * supported when C == T, or when C * P > 16384.
  */
 int blade_channelizer(const void* input, void* output, uint64_t A, uint64_t F, uint64_t T,
                       uint64_t P, uint64_t rate, blade_stream_t stream);
@@ -190,19 +192,23 @@ int blade_channelizer(const void* input, void* output, uint64_t A, uint64_t F, u
 int blade_channelizer_plan(uint64_t rate, blade_stream_t stream);
 
 /*
- * Channelizer followed by Detector in one pass (P = 2): the ModeH chain
- * (include/blade/bundles/generic/mode_h.hh:55-106: Channelizer -> Detector)
- * without the CF32 spectra ever reaching memory.  Per fine channel
+ * Channelizer, optional Polarizer and Detector in one pass (P = 2): the ModeH
+ * chain (include/blade/bundles/generic/mode_h.hh:55-106: Channelizer ->
+ * Polarizer -> Detector) without the CF32 spectra ever reaching memory.  Per
+ * fine channel
  *   out[a, f*C+k', g, q] = sum over N consecutive spectra of |X|^2+|Y|^2
  *                          (out_pols 1) or [XX*, YY*, Re XY*, Im XY*] (4),
  * as Detector (src/modules/detector/detector.cu:7-51) applied to the
- * Channelizer output.  N divides S = T/rate (output [A, F*C, S/N, out_pols]),
- * or is a multiple of S (output [A, F*C, 1, out_pols], accumulate != 0 adds
- * this block to it: stepped integration across blocks).
+ * Channelizer output; with polarize != 0, of (L, R) = (X + jY, X - jY)
+ * instead (Polarizer, polarizer.cu:5-25, between the two).  N divides
+ * S = T/rate (output [A, F*C, S/N, out_pols]), or is a multiple of S (output
+ * [A, F*C, 1, out_pols], accumulate != 0 adds this block to it: stepped
+ * integration across blocks).
  */
 int blade_channelizer_detector(const void* input, void* output, uint64_t A, uint64_t F,
                                uint64_t T, uint64_t rate, uint64_t integration,
-                               uint64_t out_pols, int accumulate, blade_stream_t stream);
+                               uint64_t out_pols, int polarize, int accumulate,
+                               blade_stream_t stream);
 
 /*
  * Polarizer XY -> LR: out[...,0] = X + jY, out[...,1] = X - jY.
@@ -258,6 +264,26 @@ int blade_phasor_ata(const void* delays, const void* calibration, void* phasors,
                      double observation_frequency_hz, double channel_bandwidth_hz,
                      double total_bandwidth_hz, uint64_t frequency_start_index,
                      blade_stream_t stream);
+
+/*
+ * Geometric delays of the ATA phasor module, on the host (no device memory,
+ * no stream).  Replaces the delay half of Modules::Phasor::ATA<OT>::process
+ * (src/modules/phasor/ata.cc:105-173, which calls radiointerferometryc99 and
+ * ERFA):
+ *   delays[b,a] = (w_a - w_ref)/c for beam b  -  (w_a - w_ref)/c for the boresight,
+ * w = cos(dec)cos(H) X - cos(dec)sin(H) Y + sin(dec) Z over the antenna's
+ * baseline (X, Y, Z) from the array reference position in the local frame
+ * (X to the meridian at the equator, Y east, Z north), dec and RA of date
+ * (J2000 precessed with the IAU 1976 angles), H = GMST(UT1, IAU 1982) +
+ * longitude - RA.  Nutation and aberration are not modelled.
+ * antenna_ecef: [A][3] metres; reference_lla: {lon, lat (rad), alt (m), WGS84};
+ * boresight_radec: {ra, dec} rad (J2000); beam_radec: [B][2] rad;
+ * julian_date: UTC; dut1: UT1 - UTC in seconds; delays: [B][A] seconds.
+ */
+int blade_ata_delays(const double* antenna_ecef, uint64_t A, uint64_t reference_antenna,
+                     const double* reference_lla, const double* boresight_radec,
+                     const double* beam_radec, uint64_t B, double julian_date, double dut1,
+                     double* delays);
 
 #ifdef __cplusplus
 }

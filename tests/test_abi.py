@@ -71,3 +71,16 @@ def test_validation_errors_mirror_reference():
     i4 = (ct.c_uint64 * 4)(0, 1, 1, 3)
     with pytest.raises(_lib.BladeError, match="repeated"):
         _lib.call("blade_permutation", None, None, 8, s4, i4, None)
+
+
+def test_run_time_compilation_without_a_device():
+    # tests/cpp/jit_compile.cc: Program::compile (hipRTC, the counterpart of
+    # the reference's NVRTC/jitify2 programs, include/blade/module.hh:22-83)
+    # turns a template instantiation over the device-side tensor types into a
+    # gfx950 code object; a source error or an unknown kernel fails.
+    import subprocess
+    exe = ROOT / "tests" / "cpp" / "_build" / "jit_compile"
+    assert exe.exists(), "tests/cpp/_build/jit_compile is missing: run the build step"
+    r = subprocess.run([str(exe)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout
+    assert "good: 0" in r.stdout and "bad: 1 log=names the file" in r.stdout

@@ -82,3 +82,18 @@ def test_runner_decorator_shape():
 
     # building a runner creates two HIP streams: only check the class wiring here
     assert P.__name__ == "P" and hasattr(P, "__call__") and hasattr(P, "copy")
+
+
+def test_blade_import_name():
+    # Reference scripts start with `import blade as bl` (e.g.
+    # tests/modules/beamformer/ata.py:2); the package of that name re-exports
+    # this one, extension submodules and `blade._internal` included.
+    import blade
+    import blade._beamformer_impl as bf
+    import blade._internal as internal
+    for name in ("module", "runner", "pipeline", "array_tensor", "phasor_tensor", "delay_tensor",
+                 "tensor", "copy", "cf32", "ci8", "ata", "modeb", "modeh", "lla", "ra_dec", "xyz"):
+        assert getattr(blade, name) is getattr(bl, name), name
+    import blade_amd._beamformer_impl as bf_amd
+    assert bf is bf_amd
+    assert internal.result.success == bl.result.success

@@ -72,13 +72,14 @@ def test_batched_matches_single_launches_and_oracle(precision):
         assert O.norm_rel_err(got[:, fs], ref) < TOL, f"block {k} {shapes[k]}"
 
 
-def test_batched_full_configuration_sampled():
+@pytest.mark.parametrize("precision", ["f32", "f16x2"])
+def test_batched_full_configuration_sampled(precision):
     # The seeded 32-block workload at full size (up to F=1024, T=8192, B=64);
     # two channels of each block against the oracle.
     from blade_amd import ops
     shapes = block_shapes(32, seed=2026)
     blocks = make_blocks(shapes, seed=5)
-    bb = ops.BatchedBeamformer(blocks, "f32")
+    bb = ops.BatchedBeamformer(blocks, precision)
     bb()
     for k, (x, w, y) in enumerate(blocks):
         fs = [k % x.shape[1], x.shape[1] - 1]

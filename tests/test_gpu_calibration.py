@@ -168,3 +168,26 @@ def test_adam_calibration_recovers_gain_phases():
     coherence = np.abs(u.sum(axis=0)) / A                                   # [F, P]
     assert losses[-1] < losses[0] * 2.0                                      # losses are negative
     assert coherence.min() > 0.98
+
+
+def test_calibration_graph_replays_match_eager_steps():
+    # The captured step (steering, fused loss and gradient, chain rule, Adam)
+    # replayed 37 times after 3 eager warm-up steps gives every step's loss
+    # and the final gains of 40 eager steps; eager torch work runs on the
+    # default stream between the replays.
+    import torch
+    from blade_amd.calibration import calibrate
+    A, F, T = 12, 4, 512
+    x, w, _ = calibrator(A, F, T, seed=9)
+    xd, wd = dev(x), dev(w)
+    g_e, _, l_e = calibrate(xd, wd, steps=40, lr=0.05, graph=False)
+    g_g, _, l_g = calibrate(xd, wd, steps=40, lr=0.05, graph=True)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(l_g, l_e, rtol=1e-5, atol=1e-7)
+    assert O.norm_rel_err(host(g_g), host(g_e)) <= 1e-5
+    # delays too (their own parameter group)
+    freqs = torch.linspace(1.0e9, 1.0015e9, F, dtype=torch.float64, device="cuda")
+    _, d_e, l_e = calibrate(xd, wd, steps=12, lr=0.05, fit_delays=True, freqs_hz=freqs, graph=False)
+    _, d_g, l_g = calibrate(xd, wd, steps=12, lr=0.05, fit_delays=True, freqs_hz=freqs, graph=True)
+    np.testing.assert_allclose(l_g, l_e, rtol=1e-5, atol=1e-7)
+    assert O.norm_rel_err(host(d_g), host(d_e)) <= 1e-5

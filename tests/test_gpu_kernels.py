@@ -112,6 +112,29 @@ def test_beamform_chain(rate, polarize, precision):
     assert O.norm_rel_err(y, ref) <= TOL
 
 
+def test_beamform_chain_production_shape():
+    # The Mode-B variant the benchmark times (no incoherent beam, rate 4,
+    # XY -> LR) at its full shape A28 F192 T8192 B16, both precisions: six
+    # channels spread over the band (and so over the grid's channel axis)
+    # against the complex128 oracle, and split FP16 against FP32 on the
+    # whole block.
+    rng = np.random.default_rng(2024)
+    A, F, T, B = 28, 192, 8192, 16
+    x8 = O.ci8_gaussian((A, F, T, 2), rng=rng)
+    w = phasors_for(B, A, F, rng).astype(np.complex64)
+    xd, wd = dev(x8), dev(w)
+    ys = {}
+    for precision in ("f32", "f16x2"):
+        y = host(ops.beamform_chain(xd, wd, False, False, 4, True, precision))
+        assert y.shape == (B, 4 * F, T // 4, 2)
+        for f in (0, 1, 63, 96, 130, 191):
+            ref = O.polarizer(O.channelizer(O.beamformer_ata(x8[:, f:f + 1], w[:, :, f:f + 1]), 4))
+            err = O.norm_rel_err(y[:, 4 * f:4 * f + 4], ref)
+            assert err <= TOL, (precision, f, err)
+        ys[precision] = y
+    assert O.norm_rel_err(ys["f16x2"], ys["f32"]) <= TOL
+
+
 @pytest.mark.parametrize("precision", ["f32", "f16x2"])
 def test_beamform_chain_caller_workspace(precision):
     # blade_beamform_chain_ws with a caller-owned workspace gives the same bits
@@ -155,7 +178,7 @@ def test_zero_size_calls_are_no_ops():
     _lib.call("blade_channelizer", null, null, 2, 3, 0, 2, 4, s)
     # T = 0 leaves no spectra per block: the integration check must not
     # divide by that count.
-    _lib.call("blade_channelizer_detector", null, null, 2, 3, 0, 8, 4, 1, 0, s)
+    _lib.call("blade_channelizer_detector", null, null, 2, 3, 0, 8, 4, 1, 0, 0, s)
     torch.cuda.synchronize()
 
 
@@ -249,6 +272,23 @@ def test_channelizer_in_place_rate_equals_t_large():
     assert O.norm_rel_err(host(t).reshape(2, 5 * 8192, 1, 2), O.channelizer(x, 8192)) <= TOL
 
 
+@pytest.mark.parametrize("A,F,T,P,C", [(1, 2, 32768, 2, 32768), (2, 1, 65536, 2, 65536),
+                                       (1, 2, 65536, 1, 65536), (1, 1, 98304, 2, 49152),
+                                       (1, 2, 2 * 20000, 2, 20000)])
+def test_channelizer_large_rates(A, F, T, P, C):
+    # Rates past one workgroup's LDS run as two passes (four-step FFT): the
+    # reference's cuFFT takes any C == T (src/modules/channelizer/base.cc:58-93),
+    # e.g. 65536 in its Mode-BH benchmark (benchmarks/bundles/ata/mode-bh/mode_bh.hh).
+    rng = np.random.default_rng(C + P)
+    x = crand(rng, A, F, T, P)
+    y = host(ops.channelizer(dev(x), C))
+    assert O.norm_rel_err(y, O.channelizer(x, C)) <= TOL
+    if T == C:   # in place
+        t = dev(x)
+        ops.channelizer(t, C, out=t)
+        assert O.norm_rel_err(host(t).reshape(A, F * C, 1, P), O.channelizer(x, C)) <= TOL
+
+
 def test_channelizer_in_place_rate_equals_t():
     rng = np.random.default_rng(3)
     x = crand(rng, 2, 3, 1024, 2)
@@ -261,13 +301,28 @@ def test_channelizer_in_place_rate_equals_t():
                                             (2, 3, 16384, 8192, 1, 4),
                                             (1, 4, 4096, 1024, 2, 1),
                                             (2, 2, 2400, 200, 3, 4)])
-def test_channelizer_detector(A, F, T, C, N, npol):
+@pytest.mark.parametrize("polarize", [False, True])
+def test_channelizer_detector(A, F, T, C, N, npol, polarize):
     rng = np.random.default_rng(C + N)
     x = crand(rng, A, F, T, 2)
-    y = host(ops.channelizer_detector(dev(x), C, N, npol))
-    ref = O.detector(O.channelizer(x, C), N, npol)
+    y = host(ops.channelizer_detector(dev(x), C, N, npol, polarize=polarize))
+    v = O.channelizer(x, C)
+    ref = O.detector(O.polarizer(v) if polarize else v, N, npol)
     assert y.shape == ref.shape
     assert O.norm_rel_err(y, ref) <= TOL
+
+
+@pytest.mark.parametrize("C", [64, 512, 4096])
+def test_channelizer_detector_polarize_stepped(C):
+    # XY -> LR before full-Stokes detection, integrated over two blocks
+    # (the three kernel families: radix-8 passes, radix 16, large C).
+    rng = np.random.default_rng(C)
+    blocks = [crand(rng, 2, 3, 2 * C, 2) for _ in range(2)]
+    out = torch.empty((2, 3 * C, 1, 4), dtype=torch.float32, device="cuda")
+    for k, b in enumerate(blocks):
+        ops.channelizer_detector(dev(b), C, 4, 4, out=out, accumulate=k > 0, polarize=True)
+    ref = O.detector_stepped([O.polarizer(O.channelizer(b, C)) for b in blocks], 4, 4)
+    assert O.norm_rel_err(host(out), ref) <= TOL
 
 
 def test_channelizer_detector_stepped():

@@ -261,15 +261,75 @@ def test_gather_and_permutation():
 
 # -------------------------------------------------------------------- bundles
 
-def test_mode_b_composition():
-    x = crand(3, 2, 8, 2)
-    w = crand(2, 3, 2 * 4, 1, 2)
-    y = O.mode_b(x, w, channelizer_rate=4, convert_to_circular=True, incoherent_beam=True,
-                 detector_enable=True, detector_integration_size=2,
-                 detector_number_of_output_polarizations=1)
-    v = O.polarizer(O.channelizer(x, 4))
-    b = O.beamformer_ata(v, w, True, True)
-    assert np.allclose(y, O.detector(b, 2, 1))
+def test_mode_b_one_antenna_unit_phasors_is_channelize_then_polarize():
+    # mode_b.hh:104-199 runs Cast -> Channelizer -> Polarizer -> Beamformer.
+    # With one antenna and w = 1 the beamformer is the identity, so the
+    # output is the polarized spectra of x: a tone e^{j2pi k0 n/C} in X only
+    # lands in fine channel (k0 + C/2) % C with amplitude C, in both L and R.
+    C, k0 = 4, 1
+    n = np.arange(16)
+    x = np.zeros((1, 2, 16, 2), complex)
+    x[0, :, :, 0] = np.exp(2j * np.pi * k0 * n / C)
+    w = np.ones((3, 1, 2 * C, 1, 2))
+    y = O.mode_b(x, w, channelizer_rate=C, convert_to_circular=True)
+    assert y.shape == (3, 2 * C, 4, 2)
+    exp = np.zeros((3, 2 * C, 4, 2), complex)
+    for f in range(2):
+        exp[:, f * C + (k0 + C // 2) % C, :, :] = C       # L = X + jY, R = X - jY, Y = 0
+    assert np.allclose(y, exp, atol=1e-12)
+    # a tone in Y: L = +jC, R = -jC
+    x2 = x[..., ::-1].copy()
+    y2 = O.mode_b(x2, w, channelizer_rate=C, convert_to_circular=True)
+    assert np.allclose(y2[..., 0], 1j * exp[..., 0], atol=1e-12)
+    assert np.allclose(y2[..., 1], -1j * exp[..., 1], atol=1e-12)
+
+
+def test_mode_b_polarizes_before_beamforming():
+    # Polarization-dependent phasors tell the two orders apart: the
+    # polarizer mixes X and Y before each pol's own phasor is applied.
+    x = crand(1, 1, 4, 2)
+    a, b = 0.5 - 2j, 1.5 + 0.25j
+    w = np.zeros((1, 1, 1, 1, 2), complex)
+    w[..., 0], w[..., 1] = a, b
+    y = O.mode_b(x, w, convert_to_circular=True)
+    X, Y = x[0, 0, :, 0], x[0, 0, :, 1]
+    assert np.allclose(y[0, 0, :, 0], a * (X + 1j * Y))
+    assert np.allclose(y[0, 0, :, 1], b * (X - 1j * Y))
+    assert not np.allclose(y[0, 0, :, 0], a * X + 1j * b * Y)     # beamform, then polarize
+
+
+def test_mode_b_detector_switches_the_incoherent_sqrt():
+    # mode_b.hh:156 sets enableIncoherentBeamSqrt = detectorEnable: with the
+    # detector the incoherent beam is sqrt(P) and Stokes I of it gives back
+    # the summed power P_X + P_Y, P_p = sum_a |x_p|^2 |w0_p|^2 (beam 0's
+    # phasors, beamformer.cu:68-69); without it the beam holds (P_p, 0).
+    x = crand(3, 2, 6, 2)
+    w = crand(2, 3, 2, 1, 2)
+    P = np.einsum("aftp,afp->ftp", np.abs(x) ** 2, np.abs(w[0, :, :, 0, :]) ** 2)
+    det = O.mode_b(x, w, incoherent_beam=True, detector_enable=True)
+    assert det.shape == (3, 2, 6, 1)
+    assert np.allclose(det[2, :, :, 0], P.sum(axis=-1))
+    # coherent beams: |sum_a x w|^2 summed over pols
+    coh = np.einsum("aftp,bafp->bftp", x, w[:, :, :, 0, :])
+    assert np.allclose(det[:2, :, :, 0], (np.abs(coh) ** 2).sum(axis=-1))
+    raw = O.mode_b(x, w, incoherent_beam=True, detector_enable=False)
+    assert raw.shape == (3, 2, 6, 2)
+    assert np.allclose(raw[2], P) and np.allclose(raw[:2], coh)
+
+
+def test_mode_h_tone_and_power():
+    # mode_h.hh:55-106: Channelizer(rate = T) -> Polarizer -> Detector.  A tone
+    # k0 in X of amplitude 1: Stokes I of (L, R) is 2|C|^2 in fine channel
+    # (k0 + T/2) % T and 0 elsewhere; without the polarizer it is |C|^2.
+    T, k0 = 16, 3
+    x = np.zeros((1, 2, T, 2), complex)
+    x[0, :, :, 0] = np.exp(2j * np.pi * k0 * np.arange(T) / T)
+    for circ, scale in ((True, 2.0), (False, 1.0)):
+        y = O.mode_h(x, convert_to_circular=circ)
+        assert y.shape == (1, 2 * T, 1, 1)
+        exp = np.zeros(2 * T)
+        exp[[(k0 + T // 2) % T, T + (k0 + T // 2) % T]] = scale * T * T
+        assert np.allclose(y[0, :, 0, 0], exp, atol=1e-9)
 
 
 def test_calibration_gradient_closed_form():
