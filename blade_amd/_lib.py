@@ -33,6 +33,8 @@ SIGNATURES = {
     "blade_beamformer_ata": [_i, _i, _p, _p, _p, _u64, _u64, _u64, _u64, _u64, _i, _i, _p],
     "blade_beamform_chain": [_i, _i, _p, _p, _p, _u64, _u64, _u64, _u64, _u64, _i, _i, _u64, _i, _p],
     "blade_beamformer_workspace_size": [_i, _u64, _u64, _u64],
+    "blade_beamform_chain_noalloc": [_i, _i, _p, _p, _p, _u64, _u64, _u64, _u64, _u64, _i, _i, _u64, _i,
+                                     _p],
     "blade_beamform_chain_ws": [_i, _i, _p, _p, _p, _u64, _u64, _u64, _u64, _u64, _i, _i, _u64, _i,
                                 _p, _u64, _p],
     "blade_beamform_batched_workspace_size": [_i, _p, _u64, _p],

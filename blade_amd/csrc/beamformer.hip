@@ -39,6 +39,8 @@
 // The accumulator of a 16x16 tile holds, for lane l and register r, row
 // 4(l/16) + r and column l%16; rows are (beam, re/im) pairs, so registers
 // (r, r+1), r even, hold one complex beam sample of time sample l%16.
+#include <mutex>
+
 #include "common.hh"
 
 #include <algorithm>
@@ -790,17 +792,17 @@ static int launch_bf(const BfArgs& args, unsigned char* ws, size_t ws_bytes, hip
     // Device scratch: only the store sink of idle lanes (the phasors are
     // packed into LDS by the kernel).
     const size_t need = kSinkBytes;
-    unsigned char* scratch = nullptr;
+    unsigned char* sink_mem = nullptr;
     if (ws) {
         if (ws_bytes < need) return fail("beamformer: workspace too small");
         if ((uintptr_t)ws % 256) return fail("beamformer: workspace must be 256-byte aligned");
-        scratch = ws;
+        sink_mem = ws;
     } else {
-        const int rc = library_workspace(WS_BEAMFORMER_SINK, need, s, &scratch);
+        const int rc = library_workspace(WS_BEAMFORMER_SINK, need, s, &sink_mem);
         if (rc) return rc;
     }
     a.pack = nullptr;
-    a.sink = reinterpret_cast<float4*>(scratch);
+    a.sink = reinterpret_cast<float4*>(sink_mem);
     a.n_tchunks = (a.T + span - 1) / span;
     // Chunks per workgroup: about 768 workgroups (one round at 3 per CU);
     // measured best for both variants at the production shape.
@@ -1118,9 +1120,63 @@ static int run_beamformer(int in_type, int precision, const void* input, const v
     return dispatch<false, false>(a, T % 4 == 0 ? 4 : 1, mw, epi, ws, wsb, s);
 }
 
+// Scratch of blade_beamform_chain_noalloc: the store sink of idle lanes as
+// a file-scope device array, kSinkBytes (1 KiB) on every device for the life
+// of the process.  In host code `scratch` names a host-side stand-in; its
+// device address, which differs per device, is looked up once per device
+// with that device current, and only that address reaches the kernels.
+// Calls that overlap on one device write the same array at the same time;
+// it is a write-only sink whose contents nothing reads, so their results do
+// not depend on each other.
+static __device__ __align__(256) unsigned char scratch[kSinkBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mutex;
+static unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+static int static_scratch(unsigned char** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        set_error(std::string("beamformer scratch: hipGetDevice: ") + hipGetErrorString(e));
+        return BLADE_HIP_ERROR;
+    }
+    if (dev < 0 || dev >= kMaxDevices) return fail("beamformer scratch: device index above 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("beamformer scratch: hipGetSymbolAddress: ") +
+                      (e != hipSuccess ? hipGetErrorString(e) : "null address"));
+            return BLADE_HIP_ERROR;
+        }
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return BLADE_SUCCESS;
+}
+
 }  // namespace bl
 
 using namespace bl;
+
+extern "C" int blade_beamform_chain_noalloc(int in_type, int precision, const void* input,
+                                            const void* phasors, void* output, uint64_t A,
+                                            uint64_t F, uint64_t T, uint64_t P, uint64_t B,
+                                            int incoherent, int incoherent_sqrt, uint64_t rate,
+                                            int polarize, blade_stream_t stream) {
+    const uint64_t need = blade_beamformer_workspace_size(precision, A, F, B);
+    if (need > kSinkBytes)
+        return fail("beamformer (no allocation): the call needs " + std::to_string(need) +
+                    " bytes of scratch, above the static " + std::to_string(kSinkBytes));
+    unsigned char* ws = nullptr;
+    const int rc = static_scratch(&ws);
+    if (rc) return rc;
+    return run_beamformer(in_type, precision, input, phasors, output, A, F, T, P, B, incoherent,
+                          incoherent_sqrt, rate, polarize, ws, kSinkBytes,
+                          reinterpret_cast<hipStream_t>(stream));
+}
 
 extern "C" int blade_beamformer_ata(int in_type, int precision, const void* input,
                                     const void* phasors, void* output, uint64_t A, uint64_t F,

@@ -104,6 +104,22 @@ int blade_beamform_chain_ws(int in_type, int precision, const void* input, const
                             blade_stream_t stream);
 
 /*
+ * blade_beamform_chain without any allocation: no reference counterpart.
+ * For callers that cannot allocate (code under hipGraph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as blade_beamform_chain; its only device scratch (the 1 KiB store
+ * sink of idle lanes, blade_beamformer_workspace_size) is a static device
+ * array of the library, one per device, held until the process ends.  Calls
+ * needing more scratch than that are rejected.  The array is write-only, so
+ * calls that overlap on one device (different streams) share it without
+ * changing each other's results.
+ */
+int blade_beamform_chain_noalloc(int in_type, int precision, const void* input, const void* phasors,
+                                 void* output, uint64_t A, uint64_t F, uint64_t T, uint64_t P,
+                                 uint64_t B, int incoherent, int incoherent_sqrt, uint64_t rate,
+                                 int polarize, blade_stream_t stream);
+
+/*
  * Many beamformer blocks of different shapes in one persistent launch (no
  * reference counterpart; each block is what one Beamformer::ATA module call
  * computes, src/modules/beamformer/beamformer.cu:13-79, without the
