@@ -321,15 +321,24 @@ def calibration_leg(steps=200):
     lam = [1.0, 0.0, 0.0, 0.0]
     _events_ms(lambda: ops.stokes_i_loss_and_grad(x, w, lam), 5)
     g_ms = _events_ms(lambda: ops.stokes_i_loss_and_grad(x, w, lam), 20)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    _, _, losses = calibrate(x, w, steps=steps, lr=0.05)
-    torch.cuda.synchronize()
-    step_ms = (time.perf_counter() - t0) * 1e3 / steps
+    def timed(n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = calibrate(x, w, steps=n, lr=0.05)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, out
+
+    timed(10)                                   # first capture, library set-up
+    t_short, _ = timed(20)
+    t_full, (_, _, losses) = timed(steps)
+    # The per-call set-up (reference loss, 3 eager steps, graph capture) is
+    # the same for both runs; the difference is what each further step costs.
+    step_ms = (t_full - t_short) * 1e3 / (steps - 20)
     bytes_ = A_ * F_ * T_ * 2 * 2
     return {"workload": "A42 F384 T2048 P2 B4, loss = -Stokes I of beam 0", "grad_ms": round(g_ms, 4),
             "grad_input_gbs": round(bytes_ / g_ms / 1e6, 1), "adam_step_ms": round(step_ms, 3),
-            "adam_steps": steps, "loss_first_last": [round(losses[0], 4), round(losses[-1], 4)]}
+            "calibrate_total_ms": round(t_full * 1e3, 1), "adam_steps": steps,
+            "loss_first_last": [round(losses[0], 4), round(losses[-1], 4)]}
 
 
 def kernels_leg(iters=20):
