@@ -89,10 +89,8 @@ struct BfArgs {
     int polarize;
     int nsteps;             // K steps (FP32: 2 per antenna group; FP16: 1 per 16 antennas)
     int n_tchunks;          // chunks of WAVES*TILE*NT samples along T
-    int chunks_per_wg;      // consecutive chunks walked by one workgroup
-    unsigned char* pack;    // packed phasor slabs, one per (f, beam group)
-    size_t slab_bytes;      // bytes per slab
-    size_t wbytes;          // bytes of A operands at the head of a slab
+    int chunks_per_wg;      // chunks walked by one workgroup
+    size_t wbytes;          // bytes of the phasor A operands in LDS
     float4* sink;           // 64 x 16 B: target of the stores of lanes with nothing to store
 };
 
@@ -110,7 +108,7 @@ struct BfArgs {
 // of the group for split FP16, whose scale is shared by the group's pol).
 template <int MW, bool F16X2, bool G3>
 __device__ __forceinline__ void bf_pack_fill(const BfArgs& a, const int m, const int p, const float2* sw,
-                                             const float scale, unsigned char* slab) {
+                                             const float scale, unsigned char* dst) {
     constexpr int AK = F16X2 ? 4 * KG : KG;
     const int A = a.A;
     const int nsteps = a.nsteps;
@@ -118,7 +116,7 @@ __device__ __forceinline__ void bf_pack_fill(const BfArgs& a, const int m, const
         // beam (within the staged rows) of A-operand row `row` of tile m
         const int row_beam0 = m * (TILE / 2);
         // [m][p][step][hl][lane] halfx8, the (m, p) block
-        halfx8* wh = reinterpret_cast<halfx8*>(slab) + (size_t)((m * 2 + p) * nsteps) * 128;
+        halfx8* wh = reinterpret_cast<halfx8*>(dst) + (size_t)((m * 2 + p) * nsteps) * 128;
         for (int e = threadIdx.x; e < nsteps * 128; e += blockDim.x) {
             const int st = e >> 7, hl = (e >> 6) & 1, ln = e & 63;
             const int row = ln & (TILE - 1), kg = ln >> 4;
@@ -151,7 +149,7 @@ __device__ __forceinline__ void bf_pack_fill(const BfArgs& a, const int m, const
         // Three-product layout [p][unit][q][g][row] float, q = (Wr, Wi, Wr + Wi),
         // row = beam of the 16-beam tile; this part is rows 8m .. 8m + 7.
         const int nunits = nsteps / 2;
-        float* wl = reinterpret_cast<float*>(slab) + (size_t)p * nunits * 3 * 64;
+        float* wl = reinterpret_cast<float*>(dst) + (size_t)p * nunits * 3 * 64;
         for (int e = threadIdx.x; e < nunits * 3 * 32; e += blockDim.x) {
             const int rl = e & 7, kg = (e >> 3) & 3, uq = e >> 5;
             const int un = uq / 3, q = uq - 3 * un;
@@ -165,7 +163,7 @@ __device__ __forceinline__ void bf_pack_fill(const BfArgs& a, const int m, const
         }
     } else {
         // [m][p][step][g][row] float, step = 2 * antenna group + component
-        float* wl = reinterpret_cast<float*>(slab) + (size_t)((m * 2 + p) * nsteps) * 64;
+        float* wl = reinterpret_cast<float*>(dst) + (size_t)((m * 2 + p) * nsteps) * 64;
         for (int e = threadIdx.x; e < nsteps * 64; e += blockDim.x) {
             const int st = e >> 6, kg = (e >> 4) & 3, row = e & (TILE - 1);
             const int an = (st >> 1) * KG + kg, c = st & 1;
@@ -190,46 +188,6 @@ __device__ __forceinline__ void bf_pack_w0sq(const BfArgs& a, const float2* sw, 
             v = w.x * w.x + w.y * w.y;
         }
         w0sq[an] = v;
-    }
-}
-
-// One (pol p, beam tile m) part of one (f, beam group), into the global
-// slab (the batched pack kernel).  The split-FP16 path stages all MW * 8
-// beams of the pol for the maximum.
-template <int MW, bool F16X2, bool G3 = (!F16X2 && MW == 2)>
-__device__ __forceinline__ void bf_pack_body(const BfArgs& a, const int mp, const int f, const int gz,
-                                             const int groups, float2* s_w, unsigned int& s_max) {
-    constexpr int NB = MW * (TILE / 2);              // beams of one group
-    constexpr int SB = F16X2 ? NB : TILE / 2;        // beams staged in LDS (s_w[SB][A], pol p)
-    const int p = mp & 1;
-    const int m = mp >> 1;
-    const int A = a.A;
-    const int b0 = gz * NB + (F16X2 ? 0 : m * (TILE / 2));   // first staged beam
-    unsigned char* slab = a.pack + ((size_t)f * groups + gz) * a.slab_bytes;
-    float* tail = reinterpret_cast<float*>(slab + a.wbytes);
-    if (threadIdx.x == 0) s_max = 0u;
-    float mx = 0.f;
-    for (int e = threadIdx.x; e < SB * A; e += blockDim.x) {
-        const int bl_ = e / A, an = e - bl_ * A, b = b0 + bl_;
-        float2 w = make_float2(0.f, 0.f);
-        if (b < a.B) w = a.w[(((size_t)b * A + an) * a.F + f) * 2 + p];
-        s_w[e] = w;
-        mx = fmaxf(mx, fmaxf(fabsf(w.x), fabsf(w.y)));
-    }
-    __syncthreads();
-    float scale = 1.f, unscale = 1.f;
-    if constexpr (F16X2) {
-        atomicMax(&s_max, __float_as_uint(mx));
-        __syncthreads();
-        int ex = 0;
-        frexpf(__uint_as_float(s_max), &ex);
-        scale = ldexpf(1.0f, -ex);
-        unscale = ldexpf(1.0f, ex);
-    }
-    bf_pack_fill<MW, F16X2, G3>(a, m, p, s_w, scale, slab);
-    if (m == 0) {
-        if (gz == 0) bf_pack_w0sq(a, s_w, tail + p * 128);
-        if (threadIdx.x == 0) tail[256 + p] = unscale;
     }
 }
 
@@ -398,6 +356,45 @@ __device__ __forceinline__ void store_beam(const BfArgs& a, int b, int f, int t0
     }
 }
 
+constexpr int EPI_MODEB = 4 | 8;          // rate 4, polarize, no incoherent beam
+constexpr int EPI_PLAIN = 1;              // rate 1
+constexpr int EPI_PLAIN_INC = 1 | 16;     // rate 1, incoherent beam
+
+// Mode-B epilogue through LDS (split FP16, 16-beam tiles): the four
+// wavefronts' spectra of one chunk are exchanged so that each store
+// instruction writes 1 KiB of one output row (64 consecutive spectra of one
+// (beam, fine channel)) instead of 256 B pieces of four rows.  Half of the
+// group's 16 beams at a time: [8 beams][4 fine channels][64 + 1] float4 in
+// LDS (the staged phasors' space, free once they are packed).
+template <int NT, bool F16X2, int MW, int EPI>
+constexpr bool staged_epilogue() { return F16X2 && MW == 2 && NT == 4 && EPI == EPI_MODEB; }
+constexpr int kStageRow = 65;                                   // float4 per staged row, padded
+constexpr size_t kStageBytes = (size_t)8 * 4 * kStageRow * 16;  // one half of the beams
+
+// One beam's 4 consecutive samples (one lane): DFT-4, fftshift, XY -> LR,
+// into the staged rows of local beam lb at column col (the spectrum's index
+// within the chunk).
+__device__ __forceinline__ void stage_beam(float4* stg, int lb, int col, float2 (&v)[2][4]) {
+    float2 o[2][4];
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+        const float2* x = &v[p][0];
+        const float2 s02 = cadd(x[0], x[2]), d02 = csub(x[0], x[2]);
+        const float2 s13 = cadd(x[1], x[3]), dd = csub(x[1], x[3]);
+        const float2 d13 = make_float2(dd.y, -dd.x);   // -j (x1 - x3)
+        o[p][0] = cadd(s02, s13);
+        o[p][1] = cadd(d02, d13);
+        o[p][2] = csub(s02, s13);
+        o[p][3] = csub(d02, d13);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        float2 X = o[0][k], Y = o[1][k];
+        polarize2(X, Y);
+        stg[(lb * 4 + ((k + 2) & 3)) * kStageRow + col] = make_float4(X.x, X.y, Y.x, Y.y);
+    }
+}
+
 // One "unit" of voltage loads: what a lane needs for one K block.
 //  FP32, CI8  : one antenna x NT samples          -> NT words
 //  FP32, CF32 : one antenna x NT samples x 16 B   -> 4 NT words
@@ -436,17 +433,13 @@ __device__ __forceinline__ void load_words(uint32_t* dst, const uint32_t* src) {
 // so the workgroups of one channel read and write adjacent pieces of each
 // antenna and beam row at the same time; with voltages coming from HBM
 // rather than the last-level cache that is ~3% faster than consecutive
-// chunk ranges.  LDS: lds_raw holds the phasor A operands, s_unscale /
-// s_w0sq the slab's tail.
-// FUSE: the workgroup packs its phasors straight into LDS (bf_pack_lds; the
-// dynamic LDS then holds the staged phasors [2][MW * 8][A] float2 after the
-// A operands) instead of copying a slab written by a pack kernel.
-template <int NT, int MW, bool CI8, bool F16X2, int D, bool NTS, int EPI, bool G3 = (!F16X2 && MW == 2),
-          bool FUSE = false>
+// chunk ranges.  The work item packs its phasors straight into LDS
+// (bf_pack_lds): lds_raw holds the A operands, then the staged phasors
+// [2][MW * 8][A] float2; s_unscale / s_w0sq the split-FP16 unscale and |w0|^2.
+template <int NT, int MW, bool CI8, bool F16X2, int D, bool NTS, int EPI, bool G3 = (!F16X2 && MW == 2)>
 __device__ __forceinline__ void bf_body(const BfArgs& a, const int f, const int gz, const int cw0,
-                                        const int groups, unsigned char* lds_raw,
-                                        float* s_unscale, float (*s_w0sq)[128], const int cstride = 1,
-                                        unsigned int* s_max = nullptr) {
+                                        unsigned char* lds_raw, float* s_unscale, float (*s_w0sq)[128],
+                                        unsigned int* s_max, const int cstride = 1) {
     constexpr int AK = F16X2 ? 4 * KG : KG;       // antennas per unit
     typedef Unit<NT, CI8, F16X2> U_t;
 
@@ -510,24 +503,9 @@ __device__ __forceinline__ void bf_body(const BfArgs& a, const int f, const int 
 
     // ---- This (f, beam group)'s packed phasors into LDS. ---------------------
     // A operands, |w0|^2 per antenna and pol, and the power-of-two unscale of
-    // the split-FP16 path: packed here, or copied from the slab that the
-    // batched pack kernel wrote.
-    if constexpr (FUSE) {
-        bf_pack_lds<MW, F16X2, G3>(a, f, gz, reinterpret_cast<float2*>(lds_raw + a.wbytes), s_max, lds_raw,
-                                   &s_w0sq[0][0], s_unscale);
-    } else {
-        const unsigned char* slab = a.pack + ((size_t)f * groups + gz) * a.slab_bytes;
-        const uint4* src = reinterpret_cast<const uint4*>(slab);
-        uint4* dst = reinterpret_cast<uint4*>(lds_raw);
-        const int n16 = (int)(a.wbytes / 16);
-        for (int e = threadIdx.x; e < n16; e += blockDim.x) dst[e] = src[e];
-        const float* tail = reinterpret_cast<const float*>(slab + a.wbytes);
-        if (do_inc) {
-            for (int e = threadIdx.x; e < 2 * 128; e += blockDim.x) (&s_w0sq[0][0])[e] = tail[e];
-        }
-        if (threadIdx.x < 2) s_unscale[threadIdx.x] = tail[256 + threadIdx.x];
-        __syncthreads();
-    }
+    // the split-FP16 path.
+    bf_pack_lds<MW, F16X2, G3>(a, f, gz, reinterpret_cast<float2*>(lds_raw + a.wbytes), s_max, lds_raw,
+                               &s_w0sq[0][0], s_unscale);
     const float unscale[2] = {s_unscale[0], s_unscale[1]};
 
     // Three-product FP32 tiles (16 beams): accumulators P1, P2, P3.
@@ -665,6 +643,42 @@ __device__ __forceinline__ void bf_body(const BfArgs& a, const int f, const int 
         // of the chunk loop (they would pin dozens of registers).
         int fo = f;
         asm volatile("" : "+s"(fo));
+        if constexpr (staged_epilogue<NT, F16X2, MW, EPI>()) {
+            // Every wavefront of the workgroup runs the same chunks, so the
+            // barriers below are reached by all four.
+            float4* stg = reinterpret_cast<float4*>(lds_raw + a.wbytes);
+            const int col = wave * TILE + i;                       // spectrum within the chunk
+            const int S = a.T / 4;
+            const int s = (cw0 + c * cstride) * (WAVES * TILE) + lane;
+            float4* const base = reinterpret_cast<float4*>(a.y);
+            float4* const sink = a.sink + lane;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                __syncthreads();   // the previous half's (or chunk's) rows are stored
+#pragma unroll
+                for (int r = 0; r < 4; r += 2) {
+                    float2 v[2][4];
+#pragma unroll
+                    for (int p = 0; p < 2; p++)
+#pragma unroll
+                        for (int n = 0; n < 4; n++)
+                            v[p][n] = make_float2(acc[h][p][n][r] * unscale[p], acc[h][p][n][r + 1] * unscale[p]);
+                    stage_beam(stg, (4 * g + r) / 2, col, v);
+                }
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const int row = wave * 8 + j;
+                    const int b = b_wg + 8 * h + (row >> 2);
+                    const float4 q = stg[row * kStageRow + lane];
+                    float4* dst = (s < S && b < a.B)
+                                      ? base + (((size_t)b * a.F + fo) * 4 + (row & 3)) * S + s
+                                      : sink;
+                    store4<NTS>(dst, make_float2(q.x, q.y), make_float2(q.z, q.w));
+                }
+            }
+            return;
+        }
         if constexpr (G3) {
             // Rows are beams: lane group g holds beams 4g .. 4g + 3.
 #pragma unroll
@@ -749,13 +763,13 @@ __device__ __forceinline__ void bf_body(const BfArgs& a, const int f, const int 
 
 // One block: blockIdx = (first chunk, f, beam group); chunks interleave.
 template <int NT, int MW, bool CI8, bool F16X2, int D, bool NTS, int EPI>
-__global__ __launch_bounds__(256, 2) void beamform_kernel(BfArgs a) {
+__global__ __launch_bounds__(256, (staged_epilogue<NT, F16X2, MW, EPI>() ? 3 : 2)) void beamform_kernel(BfArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ float s_unscale[2];
     __shared__ float s_w0sq[2][128];
     __shared__ unsigned int s_max[2];
-    bf_body<NT, MW, CI8, F16X2, D, NTS, EPI, (!F16X2 && MW == 2), true>(
-        a, blockIdx.y, blockIdx.z, blockIdx.x, gridDim.z, lds_raw, s_unscale, s_w0sq, gridDim.x, s_max);
+    bf_body<NT, MW, CI8, F16X2, D, NTS, EPI, (!F16X2 && MW == 2)>(a, blockIdx.y, blockIdx.z, blockIdx.x, lds_raw,
+                                                                s_unscale, s_w0sq, s_max, gridDim.x);
 }
 
 static int env_int(const char* name, int dflt) {
@@ -763,13 +777,12 @@ static int env_int(const char* name, int dflt) {
     return v ? atoi(v) : dflt;
 }
 
-// Bytes of one packed phasor slab and of its A-operand head.
-static void slab_sizes(bool f16x2, int mw, int nsteps, size_t* wbytes, size_t* slab, bool g3) {
-    // FP32 with two beam tiles stores three products per antenna group
-    // (Wr, Wi, Wr + Wi over 16 beams) instead of the 2 x 2 embedding rows.
-    *wbytes = f16x2 ? (size_t)mw * 2 * nsteps * 2 * 64 * 16
-                    : (g3 ? (size_t)3 * nsteps * 64 * 4 : (size_t)mw * 2 * nsteps * 64 * 4);
-    *slab = ((*wbytes + 258 * sizeof(float)) + 255) / 256 * 256;
+// Bytes of the packed phasor A operands in LDS.  FP32 with two beam tiles
+// stores three products per antenna group (Wr, Wi, Wr + Wi over 16 beams)
+// instead of the 2 x 2 embedding rows.
+static size_t operand_bytes(bool f16x2, int mw, int nsteps, bool g3) {
+    return f16x2 ? (size_t)mw * 2 * nsteps * 2 * 64 * 16
+                 : (g3 ? (size_t)3 * nsteps * 64 * 4 : (size_t)mw * 2 * nsteps * 64 * 4);
 }
 
 constexpr size_t kSinkBytes = 64 * sizeof(float4);
@@ -784,9 +797,10 @@ template <int NT, int MW, bool CI8, bool F16X2, int D, bool NTS, int EPI = 0>
 static int launch_bf(const BfArgs& args, unsigned char* ws, size_t ws_bytes, hipStream_t s) {
     BfArgs a = args;
     a.nsteps = nsteps_for(F16X2, a.A);
-    slab_sizes(F16X2, MW, a.nsteps, &a.wbytes, &a.slab_bytes, !F16X2 && MW == 2);
+    a.wbytes = operand_bytes(F16X2, MW, a.nsteps, !F16X2 && MW == 2);
     // A operands, then the staged phasors of both pols (packed in the kernel).
-    const size_t lds = a.wbytes + (size_t)2 * MW * (TILE / 2) * a.A * sizeof(float2);
+    size_t lds = a.wbytes + (size_t)2 * MW * (TILE / 2) * a.A * sizeof(float2);
+    if (staged_epilogue<NT, F16X2, MW, EPI>()) lds = std::max(lds, a.wbytes + kStageBytes);
     const int span = WAVES * TILE * NT;
     const int groups = groups_for(a.B, MW);
     // Device scratch: only the store sink of idle lanes (the phasors are
@@ -801,7 +815,6 @@ static int launch_bf(const BfArgs& args, unsigned char* ws, size_t ws_bytes, hip
         const int rc = library_workspace(WS_BEAMFORMER_SINK, need, s, &sink_mem);
         if (rc) return rc;
     }
-    a.pack = nullptr;
     a.sink = reinterpret_cast<float4*>(sink_mem);
     a.n_tchunks = (a.T + span - 1) / span;
     // Chunks per workgroup: about 768 workgroups (one round at 3 per CU);
@@ -837,9 +850,6 @@ static int launch_bf(const BfArgs& args, unsigned char* ws, size_t ws_bytes, hip
 // records (the fixed epilogues); in the generic epilogue a store instruction
 // writes 16 bytes of every 32 or 64 and ordinary stores let L2 merge the lines
 // before they go out.
-constexpr int EPI_MODEB = 4 | 8;          // rate 4, polarize, no incoherent beam
-constexpr int EPI_PLAIN = 1;              // rate 1
-constexpr int EPI_PLAIN_INC = 1 | 16;     // rate 1, incoherent beam
 
 template <int NT, int MW, bool CI8, bool F16X2, int D>
 static int launch_epi(const BfArgs& a, int epi, unsigned char* ws, size_t wsb, hipStream_t s) {
@@ -881,16 +891,14 @@ struct BatchBlock {
     const void* x;
     const float2* w;
     float2* y;
-    unsigned char* pack;    // this block's phasor slabs
-    size_t slab_bytes, wbytes;
+    size_t wbytes;          // bytes of the phasor A operands in LDS
     int A, F, T, B;
     int nsteps, n_tchunks, cpw, groups, mw;
     int item_base;          // first work item of the block
-    int pack_base;          // first pack workgroup of the block
 };
 
 struct BatchArgs {
-    int n_blocks, total_items, total_pack;
+    int n_blocks, total_items;
     float4* sink;
     BatchBlock blk[kBatchMax];
 };
@@ -902,29 +910,9 @@ __device__ __forceinline__ BfArgs batch_block_args(const BatchArgs& ba, int k) {
     a.A = b.A; a.F = b.F; a.T = b.T; a.B = b.B;
     a.incoherent = 0; a.inc_sqrt = 0; a.rate = 1; a.polarize = 0;
     a.nsteps = b.nsteps; a.n_tchunks = b.n_tchunks; a.chunks_per_wg = b.cpw;
-    a.pack = b.pack; a.slab_bytes = b.slab_bytes; a.wbytes = b.wbytes;
+    a.wbytes = b.wbytes;
     a.sink = ba.sink;
     return a;
-}
-
-template <bool F16X2>
-__global__ __launch_bounds__(256) void bf_pack_batched_kernel(BatchArgs ba) {
-    extern __shared__ float2 s_wp[];
-    __shared__ unsigned int s_max;
-    const int id = blockIdx.x;
-    int k = 0;
-    while (k + 1 < ba.n_blocks && id >= ba.blk[k + 1].pack_base) k++;
-    const BatchBlock& b = ba.blk[k];
-    int local = id - b.pack_base;
-    const int mp = local % (2 * b.mw);
-    local /= 2 * b.mw;
-    const int f = local % b.F;
-    const int gz = local / b.F;
-    const BfArgs a = batch_block_args(ba, k);
-    if (b.mw == 2)
-        bf_pack_body<2, F16X2, false>(a, mp, f, gz, b.groups, s_wp, s_max);
-    else
-        bf_pack_body<1, F16X2, false>(a, mp, f, gz, b.groups, s_wp, s_max);
 }
 
 template <int NT, bool F16X2, int D>
@@ -932,6 +920,7 @@ __global__ __launch_bounds__(256, 2) void beamform_batched_kernel(BatchArgs ba) 
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ float s_unscale[2];
     __shared__ float s_w0sq[2][128];
+    __shared__ unsigned int s_max[2];
     for (int item = blockIdx.x; item < ba.total_items; item += gridDim.x) {
         int k = 0;
         while (k + 1 < ba.n_blocks && item >= ba.blk[k + 1].item_base) k++;
@@ -945,20 +934,22 @@ __global__ __launch_bounds__(256, 2) void beamform_batched_kernel(BatchArgs ba) 
         __syncthreads();   // the previous item's readers of LDS are done
         // The four-product FP32 form here: one kernel holds both tile
         // widths, and the three-product accumulators would set its register
-        // count (and occupancy) for every block.
+        // count (and occupancy) for every block.  Each item packs its
+        // (channel, beam group)'s phasors into LDS itself, as the
+        // single-block kernel does.
         if (b.mw == 2)
-            bf_body<NT, 2, true, F16X2, D, true, EPI_PLAIN, false>(a, f, gz, tr * b.cpw, b.groups, lds_raw,
-                                                                   s_unscale, s_w0sq);
+            bf_body<NT, 2, true, F16X2, D, true, EPI_PLAIN, false>(a, f, gz, tr * b.cpw, lds_raw, s_unscale,
+                                                                   s_w0sq, s_max);
         else
-            bf_body<NT, 1, true, F16X2, D, true, EPI_PLAIN>(a, f, gz, tr * b.cpw, b.groups, lds_raw,
-                                                            s_unscale, s_w0sq);
+            bf_body<NT, 1, true, F16X2, D, true, EPI_PLAIN>(a, f, gz, tr * b.cpw, lds_raw, s_unscale, s_w0sq,
+                                                            s_max);
     }
 }
 
 struct BatchPlan {
     BatchArgs args;
-    size_t ws_bytes;        // slabs + sink
-    size_t lds, pack_lds;
+    size_t ws_bytes;        // the store sink
+    size_t lds;
     int grid;
 };
 
@@ -976,8 +967,8 @@ static int plan_batch(bool f16x2, const blade_beamform_block* blocks, int n,
     BatchArgs& ba = plan->args;
     std::memset(&ba, 0, sizeof(ba));
     ba.n_blocks = n;
-    size_t off = 0, lds = 0, pack_lds = 0;
-    long items = 0, packs = 0;
+    size_t lds = 0;
+    long items = 0;
     // Auto chunk count: about wpc * CUs * 2 items in total (two rounds).
     long all_chunks = 0;
     for (int k = 0; k < n; k++) {
@@ -1004,27 +995,21 @@ static int plan_batch(bool f16x2, const blade_beamform_block* blocks, int n,
         b.y = static_cast<float2*>(bk.output);
         b.A = (int)bk.A; b.F = (int)bk.F; b.T = (int)bk.T; b.B = (int)bk.B;
         b.nsteps = nsteps_for(f16x2, b.A);
-        slab_sizes(f16x2, b.mw, b.nsteps, &b.wbytes, &b.slab_bytes, false);
+        b.wbytes = operand_bytes(f16x2, b.mw, b.nsteps, false);
         b.groups = groups_for(b.B, b.mw);
         b.n_tchunks = (int)ceil_div(bk.T, span);
         b.cpw = std::min(b.n_tchunks, cpw_req > 0 ? cpw_req : cpw_auto);
-        b.pack = ws ? ws + off : nullptr;
-        off += ((b.slab_bytes * (size_t)b.F * b.groups) + 255) / 256 * 256;
         b.item_base = (int)items;
-        b.pack_base = (int)packs;
         items += (long)ceil_div(b.n_tchunks, b.cpw) * b.F * b.groups;
-        packs += (long)2 * b.mw * b.F * b.groups;
-        lds = std::max(lds, b.wbytes);
-        pack_lds = std::max(pack_lds, (size_t)(f16x2 ? b.mw : 1) * (TILE / 2) * b.A * sizeof(float2));
+        // A operands, then the staged phasors of both pols and the group's beams
+        lds = std::max(lds, b.wbytes + (size_t)2 * b.mw * (TILE / 2) * b.A * sizeof(float2));
     }
-    if (items > (1L << 30) || packs > (1L << 30)) return fail("batched beamformer: too many work items");
-    if (lds > 150 * 1024 || pack_lds > 150 * 1024) return fail("batched beamformer: too many antennas");
+    if (items > (1L << 30)) return fail("batched beamformer: too many work items");
+    if (lds > 150 * 1024) return fail("batched beamformer: too many antennas");
     ba.total_items = (int)items;
-    ba.total_pack = (int)packs;
-    ba.sink = ws ? reinterpret_cast<float4*>(ws + off) : nullptr;
-    plan->ws_bytes = off + kSinkBytes;
+    ba.sink = ws ? reinterpret_cast<float4*>(ws) : nullptr;
+    plan->ws_bytes = kSinkBytes;
     plan->lds = lds;
-    plan->pack_lds = pack_lds;
     plan->grid = (int)std::min<long>(items, (long)wpc * device_cus());
     return BLADE_SUCCESS;
 }
@@ -1033,17 +1018,10 @@ template <bool F16X2>
 static int launch_batch(BatchPlan& plan, hipStream_t s) {
     constexpr int D = F16X2 ? 2 : 4;
     const void* fn = reinterpret_cast<const void*>(&beamform_batched_kernel<4, F16X2, D>);
-    const void* pfn = reinterpret_cast<const void*>(&bf_pack_batched_kernel<F16X2>);
     if (plan.lds > 65536) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)plan.lds);
         if (e != hipSuccess) return fail(std::string("batched beamformer: LDS attribute: ") + hipGetErrorString(e));
     }
-    if (plan.pack_lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(pfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)plan.pack_lds);
-        if (e != hipSuccess) return fail(std::string("batched beamformer: pack LDS attribute: ") + hipGetErrorString(e));
-    }
-    hipLaunchKernelGGL(bf_pack_batched_kernel<F16X2>, dim3((unsigned)plan.args.total_pack), dim3(256),
-                       plan.pack_lds, s, plan.args);
     hipLaunchKernelGGL((beamform_batched_kernel<4, F16X2, D>), dim3((unsigned)plan.grid), dim3(64 * WAVES),
                        plan.lds, s, plan.args);
     return check_launch("batched beamformer");
@@ -1107,8 +1085,6 @@ static int run_beamformer(int in_type, int precision, const void* input, const v
     a.nsteps = 0;
     a.n_tchunks = 0;
     a.chunks_per_wg = 1;
-    a.pack = nullptr;
-    a.slab_bytes = 0;
     a.wbytes = 0;
     a.sink = nullptr;
     const int mw = B > 8 ? 2 : 1;
