@@ -307,6 +307,23 @@ def mode_a_leg(iters=20):
             "realtime_factor": round((T_ / 0.5e6) / (ms * 1e-3), 1)}
 
 
+def gradient_leg(iters=20):
+    """The fused Stokes-I loss + phasor gradient alone on A42 F384 T2048 B4
+    (the calibration leg's inner kernel), for kernel traces and counters."""
+    import torch
+    from blade_amd import ops
+    A_, F_, T_, B_ = 42, 384, 2048, 4
+    gen = torch.Generator(device="cuda").manual_seed(6)
+    x = _ci8((A_, F_, T_, 2), gen)
+    w = torch.polar(torch.ones((B_, A_, F_, 1, 2), device="cuda"),
+                    torch.rand((B_, A_, F_, 1, 2), device="cuda", generator=gen) * 6.283185307179586).to(torch.complex64)
+    lam = [1.0, 0.0, 0.0, 0.0]
+    _events_ms(lambda: ops.stokes_i_loss_and_grad(x, w, lam), 5)
+    g_ms = _events_ms(lambda: ops.stokes_i_loss_and_grad(x, w, lam), iters)
+    return {"workload": "A42 F384 T2048 P2 B4, loss = -Stokes I of beam 0", "grad_ms": round(g_ms, 4),
+            "calls": 5 + iters}
+
+
 def calibration_leg(steps=200):
     """Gain calibration on A42 F384 T2048 B4: fused Stokes-I loss + gradient
     (beams recomputed, never stored) and full Adam steps on unit-modulus gains."""
@@ -851,6 +868,22 @@ def _extra(fn):
         return {"error": f"{type(e).__name__}: {e}"}
 
 
+def _async_blit_child():
+    """The asynchronous Mode-B leg in a child process started with
+    HSA_ENABLE_SDMA=0: host<->device copies then run as kernels on the
+    Runner's streams, so one block's input copy overlaps the other's output
+    copy.  HIP reads the variable at start-up, hence the separate process."""
+    import subprocess
+    env = dict(os.environ, HSA_ENABLE_SDMA="0")
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--leg", "mode_b_bundle_async"],
+                       env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=600)
+    if r.returncode != 0:
+        return {"error": f"child exited {r.returncode}: {r.stderr.strip()[-300:]}"}
+    res = json.loads(r.stdout.strip().splitlines()[-1])["mode_b_bundle_async"]
+    return {"env": "HSA_ENABLE_SDMA=0", "ms_per_block": res.get("ms_per_block"),
+            "realtime_factor": res.get("realtime_factor"), "finite": res.get("finite")}
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as fh:
@@ -875,7 +908,7 @@ def main():
                          "plain 1 GiB device copy (blade_gather) that calibrates HBM counters")
     ap.add_argument("--leg", choices=["batched", "mode_a", "calibration", "kernels", "ingest",
                                       "reference_benchmarks", "mode_b_bundle", "mode_h_bundle",
-                                      "mode_b_bundle_async", "mode_bh_bundle", "unfused_f32"],
+                                      "mode_b_bundle_async", "mode_bh_bundle", "unfused_f32", "gradient"],
                     help="profiling mode: run only this extra leg and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -891,7 +924,8 @@ def main():
                 "kernels": kernels_leg, "ingest": ingest_leg,
                 "reference_benchmarks": reference_benchmarks_leg, "mode_b_bundle": mode_b_bundle_leg,
                 "mode_h_bundle": mode_h_bundle_leg, "mode_b_bundle_async": mode_b_bundle_async_leg,
-                "mode_bh_bundle": mode_bh_bundle_leg, "unfused_f32": unfused_leg}
+                "mode_bh_bundle": mode_bh_bundle_leg, "unfused_f32": unfused_leg,
+                "gradient": gradient_leg}
         print(json.dumps({args.leg: legs[args.leg]()}))
         return
 
@@ -963,6 +997,7 @@ def main():
         line["mode_b_bundle"] = _extra(mode_b_bundle_leg)
         line["mode_h_bundle"] = _extra(mode_h_bundle_leg)
         line["mode_b_bundle_async"] = _extra(mode_b_bundle_async_leg)
+        line["mode_b_bundle_async"]["blit_copies"] = _extra(_async_blit_child)
         line["mode_bh_bundle"] = _extra(mode_bh_bundle_leg)
     if not args.no_cpu:
         x, w, outs = bufs
