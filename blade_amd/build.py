@@ -45,7 +45,7 @@ def _run(cmd):
 def _compile(src: Path, incremental: bool) -> Path:
     obj = OBJ / (src.name + ".o")
     if incremental and obj.exists():
-        deps = [src] + list(CSRC.glob("*.hh")) + list(INCLUDE.rglob("*.h*"))
+        deps = [src] + list(CSRC.glob("*.hh")) + list((PKG / "host").glob("*.hh")) + list(INCLUDE.rglob("*.h*"))
         if all(obj.stat().st_mtime >= d.stat().st_mtime for d in deps):
             return obj
     cmd = [HIPCC] + HIP_FLAGS + ["-c", str(src), "-o", str(obj)]

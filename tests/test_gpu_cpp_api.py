@@ -60,3 +60,16 @@ def test_cpp_module_compiled_at_run_time(tmp_path):
     assert np.array_equal(y, O.polarizer(x.astype(np.complex128)).astype(np.complex64))
     d = np.fromfile(tmp_path / "d.u64", dtype=np.uint64)
     assert list(d) == [A * F * T * 2, A, F, T, 2, F * T * 2, T * 2, 2, 1]
+
+
+VEC_EXE = Path(__file__).resolve().parent / "cpp" / "_build" / "vector_semantics"
+
+
+@pytest.mark.gpu
+def test_cpp_vector_semantics_with_device_memory():
+    # The reference's Vector cases (tests/memory/vector/reference.cc:15-208)
+    # with real hipMalloc / hipHostMalloc allocations.
+    assert VEC_EXE.exists(), "tests/cpp/_build/vector_semantics is missing: run the build step"
+    r = subprocess.run([str(VEC_EXE)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout
+    assert "vector semantics: 0 failures" in r.stdout
