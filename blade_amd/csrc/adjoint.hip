@@ -10,17 +10,25 @@
 // with rows (beam, re/im), K = (time, component) and columns = antennas, on
 // v_mfma_f32_16x16x4f32 (one 16-row tile = 8 beams).
 //
-// Fused Stokes-I mode: L = -sum_b lambda_b sum_{f,t,p} |y_b|^2, so
-// gy = -lambda_b y_b.  Each workgroup recomputes its tile of y from the CI8
-// voltages already staged in LDS (the forward contraction, also on matrix
-// cores), turns it into gy in LDS and runs the adjoint on the same voltages:
-// the beams are never written to memory.
+// Fused Stokes-I mode (stokes_grad_kernel): L = -sum_b lambda_b
+// sum_{f,t,p} |y_b|^2, so gy = -lambda_b y_b.  Each workgroup recomputes its
+// tile of y from the CI8 voltages staged in LDS and runs the adjoint on the
+// same voltages: the beams are never written to memory.  Both contractions
+// are taken transposed, with rows = time for the forward product
+//     y^T[t][(b,c)] = sum_(a,c') x[a][t] . Wemb[(a,c')][(b,c)]
+// and rows = antennas for the adjoint
+//     gw^T[a][(b,c)] = sum_(t,c') x[a][t] . G[(t,c')][(b,c)],
+// so the forward accumulator a lane holds (times 4g..4g+3 of column i) is
+// exactly the adjoint's B operand for K = those four times: gy never passes
+// through LDS.  G's second component block (c' = imaginary part of x) is the
+// neighbouring column's value, one lane swap away.
 //
-// Work split: blockIdx = (time split, f, beam tile of 8).  A workgroup walks
-// its time range in tiles of ADJ_TC samples (voltages of all antennas in LDS),
-// the four waves taking a quarter of each tile; partial gradients are summed
-// across waves in LDS and across time splits by adjoint_reduce_kernel, in a
-// fixed order (deterministic, no global atomics).
+// Plain adjoint (adjoint_kernel, gy from memory): rows (beam, re/im),
+// K = (time, component), columns = antennas.
+//
+// Work split: blockIdx = (time split, f, beam tile of 8).  Partial gradients
+// are summed across waves in LDS and across time splits by
+// adjoint_reduce_kernel, in a fixed order (deterministic, no global atomics).
 #include "common.hh"
 
 #include <algorithm>
@@ -34,6 +42,8 @@ constexpr int ADJ_MAXAT = 8;       // antenna tiles of 16: A <= 128
 constexpr int ADJ_MAXB = 64;       // beams of the fused loss
 constexpr int ADJ_GYS = 17;        // s_gy row stride: 16 values + 1 word of padding
 constexpr int ADJ_BATCH = 8;       // global loads in flight per thread while staging (16 costs a wave of occupancy)
+constexpr int SG_TC = 64;          // stokes_grad_kernel: time samples per tile, 16 per wave
+constexpr int SG_XPAD = 4;         // its LDS voltage row stride is apad + 4 words (see sg_xs)
 
 // Row stride (words) of the LDS voltage tile s_x[t][antenna]: the padded
 // antenna count plus one, so the 16 lanes of a group, which read 16
@@ -58,11 +68,15 @@ __device__ __forceinline__ floatx4 adj_mfma(float a, float b, floatx4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// The value of lane l ^ 1 (DPP quad permutation [1, 0, 3, 2]).
+__device__ __forceinline__ float swap_pairs(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+
 __device__ __forceinline__ float ci8_component(uint32_t d, int idx) {
     return (float)(int8_t)((d >> (8 * idx)) & 0xFFu);
 }
 
-template <bool FUSED>
 __global__ __launch_bounds__(256) void adjoint_kernel(AdjArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int apad = a.apad;
@@ -70,8 +84,6 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjArgs a) {
     uint32_t* s_x = reinterpret_cast<uint32_t*>(lds_raw);              // [TC][xs]
     float* s_gy = reinterpret_cast<float*>(s_x + ADJ_TC * xs);          // [TC][ADJ_GYS]
     float* s_G = s_gy + ADJ_TC * ADJ_GYS;                               // [2][16][apad]
-    float* s_w = s_G + 2 * 16 * apad;                                   // [2][nsteps][64]
-    __shared__ float s_loss;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -83,53 +95,22 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjArgs a) {
     const int bt = blockIdx.z;
     const int b0 = bt * 8;
     const int A = a.A, F = a.F, T = a.T, B = a.B;
-    const int nsteps = a.nsteps;
     const int t_begin = sp * a.tsplit;
     const int t_end = min(T, t_begin + a.tsplit);
 
     for (int e = tid; e < 2 * 16 * apad; e += blockDim.x) s_G[e] = 0.f;
-    if (tid == 0) s_loss = 0.f;
-    if constexpr (FUSED) {
-        // Forward A operand of beams b0..b0+7, both pols:
-        // [p][step][g][row], step = 2 * antenna group + component.
-        // Loads in batches of ADJ_BATCH per thread, all issued before any
-        // is used, so a batch costs one memory round trip.
-        const int n_w = 2 * nsteps * 64;
-        for (int e0 = tid; e0 < n_w; e0 += ADJ_BATCH * ADJ_WAVES * 64) {
-            float2 wv[ADJ_BATCH];
-#pragma unroll
-            for (int k = 0; k < ADJ_BATCH; k++) {
-                const int e = e0 + k * ADJ_WAVES * 64;
-                const int p = e / (nsteps * 64);
-                const int r = e - p * nsteps * 64;
-                const int st = r >> 6, kg = (r >> 4) & 3, row = r & 15;
-                const int an = (st >> 1) * 4 + kg, b = b0 + (row >> 1);
-                wv[k] = (e < n_w && b < B && an < A) ? a.w[(((size_t)b * A + an) * F + f) * 2 + p]
-                                                      : make_float2(0.f, 0.f);
-            }
-#pragma unroll
-            for (int k = 0; k < ADJ_BATCH; k++) {
-                const int e = e0 + k * ADJ_WAVES * 64;
-                if (e < n_w) {
-                    const int r = e % (nsteps * 64);
-                    const int c = (r >> 6) & 1, co = r & 1;
-                    const float2 w = wv[k];
-                    s_w[e] = (co == 0) ? (c == 0 ? w.x : -w.y) : (c == 0 ? w.y : w.x);
-                }
-            }
-        }
-    }
 
     floatx4 acc2[2][ADJ_MAXAT];
 #pragma unroll
     for (int p = 0; p < 2; p++)
 #pragma unroll
         for (int at = 0; at < ADJ_MAXAT; at++) acc2[p][at] = floatx4{0.f, 0.f, 0.f, 0.f};
-    float loss = 0.f;
 
     for (int t0 = t_begin; t0 < t_end; t0 += ADJ_TC) {
         __syncthreads();   // the previous tile's readers are done
         // Voltages of every antenna for this tile, s_x[t][a] (zero-padded).
+        // Loads in batches of ADJ_BATCH per thread, all issued before any is
+        // used, so a batch costs one memory round trip.
         const int n_x = ADJ_TC * apad;
         for (int e0 = tid; e0 < n_x; e0 += ADJ_BATCH * ADJ_WAVES * 64) {
             uint32_t v[ADJ_BATCH];
@@ -150,42 +131,14 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjArgs a) {
 
 #pragma unroll
         for (int p = 0; p < 2; p++) {
-            if constexpr (FUSED) {
-                // y tile: rows (beam, re/im), columns = time; each wave owns
-                // 32 samples (two 16-column tiles).
-                floatx4 acc[2] = {floatx4{0.f, 0.f, 0.f, 0.f}, floatx4{0.f, 0.f, 0.f, 0.f}};
-                for (int st = 0; st < nsteps; st++) {
-                    const float wv = s_w[(p * nsteps + st) * 64 + lane];
-                    const int an = (st >> 1) * 4 + g, c = st & 1;
 #pragma unroll
-                    for (int n = 0; n < 2; n++) {
-                        const int t = wave * 32 + n * 16 + i;
-                        const float xv = ci8_component(s_x[t * xs + an], 2 * p + c);
-                        acc[n] = adj_mfma(wv, xv, acc[n]);
-                    }
-                }
-                // lane (g, i), register r: row 4g + r = (beam 2g + r/2, comp r&1)
-#pragma unroll
-                for (int n = 0; n < 2; n++) {
-                    const int t = wave * 32 + n * 16 + i;
-#pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const int b = b0 + 2 * g + (r >> 1);
-                        const float lam = b < B ? a.lambda[b] : 0.f;
-                        loss += lam * acc[n][r] * acc[n][r];
-                        s_gy[t * ADJ_GYS + 4 * g + r] = -lam * acc[n][r];
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int e = tid; e < ADJ_TC * 8; e += ADJ_WAVES * 64) {
-                    const int bb = e / ADJ_TC, t = e - bb * ADJ_TC;
-                    const int b = b0 + bb, tt = t0 + t;
-                    float2 v = make_float2(0.f, 0.f);
-                    if (b < B && tt < t_end) v = a.gy[(((size_t)b * F + f) * T + tt) * 2 + p];
-                    s_gy[t * ADJ_GYS + 2 * bb] = v.x;
-                    s_gy[t * ADJ_GYS + 2 * bb + 1] = v.y;
-                }
+            for (int e = tid; e < ADJ_TC * 8; e += ADJ_WAVES * 64) {
+                const int bb = e / ADJ_TC, t = e - bb * ADJ_TC;
+                const int b = b0 + bb, tt = t0 + t;
+                float2 v = make_float2(0.f, 0.f);
+                if (b < B && tt < t_end) v = a.gy[(((size_t)b * F + f) * T + tt) * 2 + p];
+                s_gy[t * ADJ_GYS + 2 * bb] = v.x;
+                s_gy[t * ADJ_GYS + 2 * bb + 1] = v.y;
             }
             __syncthreads();
 
@@ -222,10 +175,6 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjArgs a) {
                     atomicAdd(&s_G[(p * 16 + 4 * g + r) * apad + at * 16 + i], acc2[p][at][r]);
             }
         }
-    if constexpr (FUSED) {
-        for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off, 64);
-        if (lane == 0) atomicAdd(&s_loss, loss);
-    }
     __syncthreads();
     float* out = a.partial + ((((size_t)sp * F + f) * 2) * a.btiles + bt) * 16 * apad;
     const size_t pstride = (size_t)a.btiles * 16 * apad;   // between pols
@@ -233,9 +182,179 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjArgs a) {
         const int p = e / (16 * apad), r = e - p * 16 * apad;
         out[p * pstride + r] = s_G[e];
     }
-    if constexpr (FUSED) {
-        if (tid == 0) a.loss_part[((size_t)sp * F + f) * a.btiles + bt] = s_loss;
+}
+
+// LDS row stride (words) of stokes_grad_kernel's voltage tile s_x[t][antenna]:
+// apad + 4 puts both read patterns on distinct banks, the forward's (16 lanes
+// read 16 consecutive times of one antenna; lane groups take adjacent
+// antennas) and the adjoint's (16 lanes read 16 consecutive antennas at one
+// time; lane groups take times 4 apart).
+__host__ __device__ constexpr int sg_xs(int apad) { return apad + SG_XPAD; }
+
+// Fused Stokes-I loss and phasor gradient for A <= 16 NAT antennas.  Each
+// wave takes 16 time rows of every SG_TC-sample tile; the next tile's
+// voltages are loaded into registers while this one is computed and written
+// to the other half of a double-buffered LDS stage: one barrier per tile.
+template <int NAT>
+__global__ __launch_bounds__(256, (NAT <= 4 ? 4 : 2)) void stokes_grad_kernel(AdjArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    constexpr int APAD = 16 * NAT;
+    constexpr int XS = sg_xs(APAD);
+    constexpr int PER = SG_TC * APAD / (64 * ADJ_WAVES);     // staged dwords per thread
+    uint32_t* s_x = reinterpret_cast<uint32_t*>(lds_raw);     // [2][SG_TC][XS]
+    float* s_w = reinterpret_cast<float*>(s_x + 2 * SG_TC * XS);   // [2][nsteps][64]
+    __shared__ float s_loss[ADJ_WAVES];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int i = lane & 15;
+    const int g = lane >> 4;
+    const int sp = blockIdx.x;
+    const int f = blockIdx.y;
+    const int bt = blockIdx.z;
+    const int b0 = bt * 8;
+    const int A = a.A, F = a.F, T = a.T, B = a.B;
+    const int nsteps = a.nsteps;
+    const int n_ag = nsteps / 2;                              // antenna groups of 4
+    const int t_begin = sp * a.tsplit;
+    const int t_end = min(T, t_begin + a.tsplit);
+    const int ntiles = (t_end - t_begin + SG_TC - 1) / SG_TC;
+
+    // Voltage staging: thread (wave, lane) takes antennas wave + 4k at time
+    // lane of the tile (one coalesced 256-byte row per wave and k).
+    uint32_t pre[PER];
+    auto load_tile = [&](int t0) {
+        const int tt = t0 + lane;
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int an = wave + ADJ_WAVES * k;
+            pre[k] = (an < A && tt < t_end) ? a.x[((size_t)an * F + f) * T + tt] : 0u;
+        }
+    };
+    auto store_tile = [&](uint32_t* dst) {
+#pragma unroll
+        for (int k = 0; k < PER; k++) dst[lane * XS + wave + ADJ_WAVES * k] = pre[k];
+    };
+    load_tile(t_begin);
+
+    // Forward B operand of beams b0..b0+7, both pols: [p][step][g][col],
+    // step = 2 * antenna group + component of x, col = (beam, re/im).
+    {
+        const int n_w = 2 * nsteps * 64;
+        for (int e0 = tid; e0 < n_w; e0 += ADJ_BATCH * ADJ_WAVES * 64) {
+            float2 wv[ADJ_BATCH];
+#pragma unroll
+            for (int k = 0; k < ADJ_BATCH; k++) {
+                const int e = e0 + k * ADJ_WAVES * 64;
+                const int p = e / (nsteps * 64);
+                const int r = e - p * nsteps * 64;
+                const int st = r >> 6, kg = (r >> 4) & 3, col = r & 15;
+                const int an = (st >> 1) * 4 + kg, b = b0 + (col >> 1);
+                wv[k] = (e < n_w && b < B && an < A) ? a.w[(((size_t)b * A + an) * F + f) * 2 + p]
+                                                      : make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (int k = 0; k < ADJ_BATCH; k++) {
+                const int e = e0 + k * ADJ_WAVES * 64;
+                if (e < n_w) {
+                    const int r = e % (nsteps * 64);
+                    const int c = (r >> 6) & 1, co = r & 1;
+                    const float2 w = wv[k];
+                    // y_r = x_r w_r - x_i w_i, y_i = x_r w_i + x_i w_r
+                    s_w[e] = (co == 0) ? (c == 0 ? w.x : -w.y) : (c == 0 ? w.y : w.x);
+                }
+            }
+        }
     }
+    store_tile(s_x);
+    if (ntiles > 1) load_tile(t_begin + SG_TC);
+    __syncthreads();
+
+    const int bcol = b0 + (i >> 1);
+    const float lam = bcol < B ? a.lambda[bcol] : 0.f;
+    const float swap_sign = (i & 1) ? -1.f : 1.f;
+    floatx4 acc2[2][NAT];
+#pragma unroll
+    for (int p = 0; p < 2; p++)
+#pragma unroll
+        for (int at = 0; at < NAT; at++) acc2[p][at] = floatx4{0.f, 0.f, 0.f, 0.f};
+    float loss = 0.f;
+
+    for (int k = 0; k < ntiles; k++) {
+        const uint32_t* sx = s_x + (k & 1) * SG_TC * XS;
+        const int trow = wave * 16;
+        // Forward: y^T rows = this wave's 16 times, both pols from one dword.
+        floatx4 y[2] = {floatx4{0.f, 0.f, 0.f, 0.f}, floatx4{0.f, 0.f, 0.f, 0.f}};
+        for (int ag = 0; ag < n_ag; ag++) {
+            const uint32_t d = sx[(trow + i) * XS + ag * 4 + g];
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                const float w0 = s_w[(p * nsteps + 2 * ag) * 64 + lane];
+                const float w1 = s_w[(p * nsteps + 2 * ag + 1) * 64 + lane];
+                y[p] = adj_mfma(ci8_component(d, 2 * p), w0, y[p]);
+                y[p] = adj_mfma(ci8_component(d, 2 * p + 1), w1, y[p]);
+            }
+        }
+        // Lane (g, i), register r: y at time trow + 4g + r, column i.  Its
+        // gradient is gy = -lambda y; the imaginary-x block of G takes the
+        // neighbouring column (re <-> im) with the sign of conj(x).
+        float gyo[2][4], gys[2][4];
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const float v = y[p][r];
+                loss += lam * v * v;
+                gyo[p][r] = -lam * v;
+                gys[p][r] = swap_sign * swap_pairs(gyo[p][r]);
+            }
+        // Adjoint: K = times trow + 4g + r (g = k index), both x components.
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const uint32_t* row = sx + (trow + 4 * g + r) * XS + i;
+#pragma unroll
+            for (int at = 0; at < NAT; at++) {
+                const uint32_t d = row[at * 16];
+#pragma unroll
+                for (int p = 0; p < 2; p++) {
+                    acc2[p][at] = adj_mfma(ci8_component(d, 2 * p), gyo[p][r], acc2[p][at]);
+                    acc2[p][at] = adj_mfma(ci8_component(d, 2 * p + 1), gys[p][r], acc2[p][at]);
+                }
+            }
+        }
+        if (k + 1 < ntiles) {
+            // Tile k + 1 into the other buffer (last read in iteration k - 1,
+            // before the barrier that ended it), then tile k + 2's loads.
+            store_tile(s_x + ((k + 1) & 1) * SG_TC * XS);
+            if (k + 2 < ntiles) load_tile(t_begin + (k + 2) * SG_TC);
+        }
+        __syncthreads();
+    }
+
+    // Partials of the four waves into LDS (the voltage stage is free now),
+    // [wave][p][col][antenna], then summed in wave order.
+    float* s_part = reinterpret_cast<float*>(lds_raw);
+#pragma unroll
+    for (int p = 0; p < 2; p++)
+#pragma unroll
+        for (int at = 0; at < NAT; at++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                s_part[((wave * 2 + p) * 16 + i) * APAD + at * 16 + 4 * g + r] = acc2[p][at][r];
+    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off, 64);
+    if (lane == 0) s_loss[wave] = loss;
+    __syncthreads();
+    constexpr int NPART = 2 * 16 * APAD;
+    float* out = a.partial + ((((size_t)sp * F + f) * 2) * a.btiles + bt) * 16 * APAD;
+    const size_t pstride = (size_t)a.btiles * 16 * APAD;   // between pols
+    for (int e = tid; e < NPART; e += blockDim.x) {
+        const float v = ((s_part[e] + s_part[NPART + e]) + s_part[2 * NPART + e]) + s_part[3 * NPART + e];
+        const int p = e / (16 * APAD), r = e - p * 16 * APAD;
+        out[p * pstride + r] = v;
+    }
+    if (tid == 0)
+        a.loss_part[((size_t)sp * F + f) * a.btiles + bt] = ((s_loss[0] + s_loss[1]) + s_loss[2]) + s_loss[3];
 }
 
 // gw[b,a,f,p] = sum over time splits of the partial rows (b, re) and (b, im).
@@ -299,13 +418,29 @@ static int run_adjoint(bool fused, const void* input, const void* phasors, const
     a.btiles = (int)ceil_div(B, 8);
     if (fused)
         for (uint64_t b = 0; b < B; b++) a.lambda[b] = beam_weights[b];
-    // Time splits: about 2048 workgroups, each a whole number of tiles.
-    const long tiles = (long)ceil_div(std::max<uint64_t>(T, 1), ADJ_TC);
+    // Time splits, each a whole number of tiles.  Plain adjoint: about 2048
+    // workgroups.  Fused: the split length that minimises rounds of
+    // 4 resident workgroups per CU x tiles per workgroup (longest on ties,
+    // which keeps the partial sums few).
+    const int tc = fused ? SG_TC : ADJ_TC;
+    const long tiles = (long)ceil_div(std::max<uint64_t>(T, 1), (uint64_t)tc);
     const long per_split = (long)F * a.btiles;
-    long nsplit = std::max(1L, std::min(tiles, (2048 + per_split - 1) / per_split));
-    const long tiles_per = (tiles + nsplit - 1) / nsplit;
-    nsplit = (tiles + tiles_per - 1) / tiles_per;
-    a.tsplit = (int)(tiles_per * ADJ_TC);
+    long tiles_per;
+    if (fused) {
+        const long slots = 256L * (a.n_at <= 4 ? 4 : 2);
+        long best = -1;
+        tiles_per = tiles;
+        for (long tp = tiles; tp >= 1; tp--) {
+            const long ns = (tiles + tp - 1) / tp;
+            const long cost = ((ns * per_split + slots - 1) / slots) * tp;
+            if (best < 0 || cost < best) { best = cost; tiles_per = tp; }
+        }
+    } else {
+        const long ns = std::max(1L, std::min(tiles, (2048 + per_split - 1) / per_split));
+        tiles_per = (tiles + ns - 1) / ns;
+    }
+    const long nsplit = (tiles + tiles_per - 1) / tiles_per;
+    a.tsplit = (int)(tiles_per * tc);
 
     const size_t part_floats = (size_t)nsplit * F * 2 * a.btiles * 16 * a.apad;
     const size_t loss_floats = (size_t)nsplit * F * a.btiles;
@@ -317,10 +452,23 @@ static int run_adjoint(bool fused, const void* input, const void* phasors, const
     a.partial = reinterpret_cast<float*>(ws);
     a.loss_part = a.partial + part_floats;
 
-    const size_t lds = ((size_t)ADJ_TC * adj_xs(a.apad) + ADJ_TC * ADJ_GYS + 2 * 16 * a.apad +
-                        (fused ? 2 * (size_t)a.nsteps * 64 : 0)) * 4;
-    const void* fn = fused ? reinterpret_cast<const void*>(&adjoint_kernel<true>)
-                           : reinterpret_cast<const void*>(&adjoint_kernel<false>);
+    const size_t lds = fused ? ((size_t)2 * SG_TC * sg_xs(a.apad) + 2 * (size_t)a.nsteps * 64) * 4
+                             : ((size_t)ADJ_TC * adj_xs(a.apad) + ADJ_TC * ADJ_GYS + 2 * 16 * a.apad) * 4;
+    const void* fn = nullptr;
+    if (fused) {
+        switch (a.n_at) {
+            case 1: fn = reinterpret_cast<const void*>(&stokes_grad_kernel<1>); break;
+            case 2: fn = reinterpret_cast<const void*>(&stokes_grad_kernel<2>); break;
+            case 3: fn = reinterpret_cast<const void*>(&stokes_grad_kernel<3>); break;
+            case 4: fn = reinterpret_cast<const void*>(&stokes_grad_kernel<4>); break;
+            case 5: fn = reinterpret_cast<const void*>(&stokes_grad_kernel<5>); break;
+            case 6: fn = reinterpret_cast<const void*>(&stokes_grad_kernel<6>); break;
+            case 7: fn = reinterpret_cast<const void*>(&stokes_grad_kernel<7>); break;
+            default: fn = reinterpret_cast<const void*>(&stokes_grad_kernel<8>); break;
+        }
+    } else {
+        fn = reinterpret_cast<const void*>(&adjoint_kernel);
+    }
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) {
@@ -329,10 +477,20 @@ static int run_adjoint(bool fused, const void* input, const void* phasors, const
         }
     }
     dim3 grid((unsigned)nsplit, (unsigned)F, (unsigned)a.btiles);
-    if (fused)
-        hipLaunchKernelGGL(adjoint_kernel<true>, grid, dim3(64 * ADJ_WAVES), lds, s, a);
-    else
-        hipLaunchKernelGGL(adjoint_kernel<false>, grid, dim3(64 * ADJ_WAVES), lds, s, a);
+    if (fused) {
+        switch (a.n_at) {
+            case 1: hipLaunchKernelGGL(stokes_grad_kernel<1>, grid, dim3(64 * ADJ_WAVES), lds, s, a); break;
+            case 2: hipLaunchKernelGGL(stokes_grad_kernel<2>, grid, dim3(64 * ADJ_WAVES), lds, s, a); break;
+            case 3: hipLaunchKernelGGL(stokes_grad_kernel<3>, grid, dim3(64 * ADJ_WAVES), lds, s, a); break;
+            case 4: hipLaunchKernelGGL(stokes_grad_kernel<4>, grid, dim3(64 * ADJ_WAVES), lds, s, a); break;
+            case 5: hipLaunchKernelGGL(stokes_grad_kernel<5>, grid, dim3(64 * ADJ_WAVES), lds, s, a); break;
+            case 6: hipLaunchKernelGGL(stokes_grad_kernel<6>, grid, dim3(64 * ADJ_WAVES), lds, s, a); break;
+            case 7: hipLaunchKernelGGL(stokes_grad_kernel<7>, grid, dim3(64 * ADJ_WAVES), lds, s, a); break;
+            default: hipLaunchKernelGGL(stokes_grad_kernel<8>, grid, dim3(64 * ADJ_WAVES), lds, s, a); break;
+        }
+    } else {
+        hipLaunchKernelGGL(adjoint_kernel, grid, dim3(64 * ADJ_WAVES), lds, s, a);
+    }
     const size_t n = (size_t)B * A * F * 2;
     hipLaunchKernelGGL(adjoint_reduce_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, a,
                        (int)nsplit, static_cast<float2*>(grad_phasors));
