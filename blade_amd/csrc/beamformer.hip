@@ -932,14 +932,12 @@ __global__ __launch_bounds__(256, 2) void beamform_batched_kernel(BatchArgs ba) 
         const int tr = local / b.F;
         const BfArgs a = batch_block_args(ba, k);
         __syncthreads();   // the previous item's readers of LDS are done
-        // The four-product FP32 form here: one kernel holds both tile
-        // widths, and the three-product accumulators would set its register
-        // count (and occupancy) for every block.  Each item packs its
-        // (channel, beam group)'s phasors into LDS itself, as the
-        // single-block kernel does.
+        // FP32 16-beam tiles take the three-product form, as in the
+        // single-block kernel.  Each item packs its (channel, beam group)'s
+        // phasors into LDS itself, as the single-block kernel does.
         if (b.mw == 2)
-            bf_body<NT, 2, true, F16X2, D, true, EPI_PLAIN, false>(a, f, gz, tr * b.cpw, lds_raw, s_unscale,
-                                                                   s_w0sq, s_max);
+            bf_body<NT, 2, true, F16X2, D, true, EPI_PLAIN, !F16X2>(a, f, gz, tr * b.cpw, lds_raw, s_unscale,
+                                                                    s_w0sq, s_max);
         else
             bf_body<NT, 1, true, F16X2, D, true, EPI_PLAIN>(a, f, gz, tr * b.cpw, lds_raw, s_unscale, s_w0sq,
                                                             s_max);
@@ -995,7 +993,7 @@ static int plan_batch(bool f16x2, const blade_beamform_block* blocks, int n,
         b.y = static_cast<float2*>(bk.output);
         b.A = (int)bk.A; b.F = (int)bk.F; b.T = (int)bk.T; b.B = (int)bk.B;
         b.nsteps = nsteps_for(f16x2, b.A);
-        b.wbytes = operand_bytes(f16x2, b.mw, b.nsteps, false);
+        b.wbytes = operand_bytes(f16x2, b.mw, b.nsteps, !f16x2 && b.mw == 2);
         b.groups = groups_for(b.B, b.mw);
         b.n_tchunks = (int)ceil_div(bk.T, span);
         b.cpw = std::min(b.n_tchunks, cpw_req > 0 ? cpw_req : cpw_auto);

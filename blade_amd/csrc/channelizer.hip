@@ -904,6 +904,12 @@ __device__ __forceinline__ void halves(float x, float& u, float& v) {
     v = __uint_as_float(p[1]);
 }
 
+// Workgroup order with consecutive values on one XCD: the dispatcher deals
+// workgroups to the 8 XCDs round-robin (b mod 8), so b -> (b mod 8) G/8 + b/8
+// hands workgroups that take consecutive spectra to one XCD, whose L2 then
+// merges their interleaved pieces of each output line.  G is a multiple of 8.
+__device__ __forceinline__ unsigned xcd_order(unsigned b, unsigned G) { return (b & 7u) * (G >> 3) + (b >> 3); }
+
 // SM (MODE 0 stores): 0 strided, 1 nontemporal with one spectrum per
 // (antenna, channel), 2 transposed through LDS so that the workgroup's SPW
 // consecutive spectra of one (antenna, channel) land as whole lines.
@@ -962,6 +968,8 @@ __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__
         else return ((i / R) * TPS + (i % R) * NSL + (C >> 1)) & (C - 1);
     };
 
+    // Strided stores: workgroups of consecutive spectra share an XCD.
+    const unsigned wgi = (MODE == 0 && SM == 0) ? xcd_order(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint64_t per_round = (uint64_t)gridDim.x * SPW;
     const uint64_t rounds = (n_items + per_round - 1) / per_round;
     const int nspec = MODE == 0 ? 1 : (int)S;
@@ -974,7 +982,7 @@ __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__
     const unsigned SC = (unsigned)(MODE == 0 ? 1 : S) * C;      // float4s per item
     float4 q[16];
     auto load = [&](uint64_t round, int s) {
-        const uint64_t first = (round * gridDim.x + blockIdx.x) * SPW;
+        const uint64_t first = (round * gridDim.x + wgi) * SPW;
         const bool any = first < n_items;
         const uint64_t b0 = any ? (MODE == 0 ? first * C : (first * S + s) * C) : 0;
         const unsigned lo = ((any && first + slot < n_items) ? slot * SC : 0u) + tl;
@@ -998,7 +1006,7 @@ __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__
         __builtin_amdgcn_s_waitcnt(0);
     }
     for (uint64_t round = 0; round < rounds; round++) {
-        const uint64_t first = (round * gridDim.x + blockIdx.x) * SPW;
+        const uint64_t first = (round * gridDim.x + wgi) * SPW;
         const uint64_t item = first + slot;
         const bool valid = item < n_items;
         for (int s = 0; s < nspec; s++) {
@@ -1283,7 +1291,8 @@ static int launch_fft16_t(const float2* in, void* out, uint64_t S, uint64_t n_it
     const uint64_t nb = ceil_div(n_items, SPW);
     const uint64_t resident = (uint64_t)device_cus() * (NT >= 512 ? 1 : 2);
     const uint64_t rounds = ceil_div(nb, resident);
-    const uint64_t blocks = ceil_div(nb, rounds);      // equal rounds for every workgroup
+    uint64_t blocks = ceil_div(nb, rounds);            // equal rounds for every workgroup
+    if (MODE == 0 && SM == 0) blocks = ceil_div(blocks, 8) * 8;   // xcd_order needs G % 8 == 0
     float4* sink = reinterpret_cast<float4*>(const_cast<float2*>(tw) + C);
     hipLaunchKernelGGL((fft16_kernel<LC, NT, MODE, SM>), dim3((unsigned)blocks), dim3(NT), lds, s,
                        reinterpret_cast<const float4*>(in), out, S, n_items, N, accumulate, tw, sink);
