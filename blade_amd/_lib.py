@@ -40,7 +40,7 @@ SIGNATURES = {
     "blade_beamform_batched_workspace_size": [_i, _p, _u64, _p],
     "blade_beamform_batched": [_i, _p, _u64, _p, _p, _u64, _p],
     "blade_beamformer_adjoint": [_p, _p, _p, _u64, _u64, _u64, _u64, _u64, _p],
-    "blade_beamformer_stokes_i_grad": [_p, _p, _p, _p, _p, _u64, _u64, _u64, _u64, _u64, _p],
+    "blade_beamformer_stokes_i_grad": [_i, _p, _p, _p, _p, _p, _u64, _u64, _u64, _u64, _u64, _p],
     "blade_channelizer": [_p, _p, _u64, _u64, _u64, _u64, _u64, _p],
     "blade_channelizer_plan": [_u64, _p],
     "blade_channelizer_detector": [_p, _p, _u64, _u64, _u64, _u64, _u64, _u64, _i, _i, _p],

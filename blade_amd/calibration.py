@@ -51,11 +51,13 @@ class Beamform(torch.autograd.Function):
 
 
 class StokesIPower(torch.autograd.Function):
-    """``L = -sum_b beam_weights[b] sum_{f,t,p} |y_b|^2`` (FP64 scalar)."""
+    """``L = -sum_b beam_weights[b] sum_{f,t,p} |y_b|^2`` (FP64 scalar).
+
+    ``precision``: "f32" or "f16x2" (FP16 matrix cores with hi + lo halves)."""
 
     @staticmethod
-    def forward(ctx, x, w, beam_weights):
-        loss, grad = ops.stokes_i_loss_and_grad(x, w.contiguous(), beam_weights)
+    def forward(ctx, x, w, beam_weights, precision="f32"):
+        loss, grad = ops.stokes_i_loss_and_grad(x, w.contiguous(), beam_weights, precision=precision)
         ctx.save_for_backward(grad)
         return loss
 
@@ -63,17 +65,17 @@ class StokesIPower(torch.autograd.Function):
     def backward(ctx, grad_loss):
         (grad,) = ctx.saved_tensors
         # PyTorch's complex gradient is twice dL/dconj(w).
-        return None, grad * (2.0 * grad_loss.to(torch.float32)), None
+        return None, grad * (2.0 * grad_loss.to(torch.float32)), None, None
 
 
 def beamform(x: torch.Tensor, w: torch.Tensor) -> torch.Tensor:
     return Beamform.apply(x, w)
 
 
-def stokes_i_power(x: torch.Tensor, w: torch.Tensor, beam_weights=None) -> torch.Tensor:
+def stokes_i_power(x: torch.Tensor, w: torch.Tensor, beam_weights=None, precision: str = "f32") -> torch.Tensor:
     if beam_weights is None:
         beam_weights = [1.0] + [0.0] * (w.shape[0] - 1)
-    return StokesIPower.apply(x, w, list(beam_weights))
+    return StokesIPower.apply(x, w, list(beam_weights), precision)
 
 
 def steer_phasors(w: torch.Tensor, gains: torch.Tensor | None, delays: torch.Tensor | None,
@@ -92,7 +94,8 @@ def steer_phasors(w: torch.Tensor, gains: torch.Tensor | None, delays: torch.Ten
 
 def calibrate(x: torch.Tensor, w: torch.Tensor, steps: int = 200, lr: float = 0.05,
               beam_weights=None, fit_gains: bool = True, fit_delays: bool = False,
-              freqs_hz: torch.Tensor | None = None, delay_lr: float = 1e-10, graph: bool = True):
+              freqs_hz: torch.Tensor | None = None, delay_lr: float = 1e-10, graph: bool = True,
+              precision: str = "f32"):
     """Adam on unit-modulus gains [A,F,P] (and delays [A]) maximising beam power.
 
     Returns ``(gains, delays, losses)``; the loss is normalised by its value at
@@ -100,6 +103,7 @@ def calibrate(x: torch.Tensor, w: torch.Tensor, steps: int = 200, lr: float = 0.
     stay on the device until the end (one read).  With ``graph`` the step
     (steering, fused loss and gradient, chain rule, Adam) is captured once in
     a CUDA graph after a few eager steps on the same stream and replayed.
+    ``precision`` selects the loss + gradient kernel ("f32" or "f16x2").
     """
     A, F, P = w.shape[1], w.shape[2], w.shape[4]
     params = []
@@ -119,11 +123,11 @@ def calibrate(x: torch.Tensor, w: torch.Tensor, steps: int = 200, lr: float = 0.
     if steps == 0:
         return gains, delays, []
     with torch.no_grad():
-        ref = abs(float(stokes_i_power(x, steer_phasors(w, gains, delays, freqs_hz), beam_weights)))
+        ref = abs(float(stokes_i_power(x, steer_phasors(w, gains, delays, freqs_hz), beam_weights, precision)))
     inv = 1.0 / ref if ref > 0 else 1.0
 
     def step():
-        loss = stokes_i_power(x, steer_phasors(w, gains, delays, freqs_hz), beam_weights)
+        loss = stokes_i_power(x, steer_phasors(w, gains, delays, freqs_hz), beam_weights, precision)
         (loss * inv).backward()
         opt.step()
         return loss.detach() * inv

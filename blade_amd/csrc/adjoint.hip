@@ -357,25 +357,260 @@ __global__ __launch_bounds__(256, (NAT <= 4 ? 4 : 2)) void stokes_grad_kernel(Ad
         a.loss_part[((size_t)sp * F + f) * a.btiles + bt] = ((s_loss[0] + s_loss[1]) + s_loss[2]) + s_loss[3];
 }
 
+// Power-of-two scale s with m * s in [0.5, 1) for m > 0 (1 for m = 0), and 1/s.
+__device__ __forceinline__ void pow2_scale(float m, float& s, float& inv) {
+    int ex = 0;
+    frexpf(m, &ex);
+    s = m > 0.f ? ldexpf(1.0f, -ex) : 1.f;
+    inv = m > 0.f ? ldexpf(1.0f, ex) : 1.f;
+}
+
+// v (|v| < 1) as hi + lo FP16 halves, into element j of two vectors.
+__device__ __forceinline__ void split_f16(float v, halfx8& hi, halfx8& lo, int j) {
+    const _Float16 h = (_Float16)v;
+    hi[j] = h;
+    lo[j] = (_Float16)(v - (float)h);
+}
+
+// Split-FP16 form of stokes_grad_kernel: the same transposed contractions and
+// tiles on v_mfma_f32_16x16x32_f16 (K = 32 per instruction), FP32
+// accumulation.  CI8 voltages are exact in FP16.
+//   * Forward B operand: the phasor embedding times a power-of-two scale per
+//     pol (every component below 1), as hi + lo FP16 halves; K = 16
+//     antennas x (re, im) of x per instruction.
+//   * Adjoint B operand: the wave's gy of the tile, K = 16 times x (re, im)
+//     -- exactly the forward outputs the lane holds (times 4g..4g+3 and the
+//     neighbouring column) -- scaled by a power of two from the wave's
+//     largest |gy| and split the same way; each tile's product is unscaled
+//     into the FP32 sums.
+// Error against FP64 is that of FP32 arithmetic (hi + lo carry ~22 bits).
+template <int NAT>
+__global__ __launch_bounds__(256, (NAT <= 3 ? 3 : 2)) void stokes_grad_f16_kernel(AdjArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    constexpr int APAD = 16 * NAT;
+    constexpr int XS = sg_xs(APAD);
+    constexpr int PER = SG_TC * APAD / (64 * ADJ_WAVES);
+    uint32_t* s_x = reinterpret_cast<uint32_t*>(lds_raw);                 // [2][SG_TC][XS]
+    halfx8* s_wh = reinterpret_cast<halfx8*>(s_x + 2 * SG_TC * XS);       // [p][blk][hl][64]
+    __shared__ float s_loss[ADJ_WAVES];
+    __shared__ unsigned int s_wmax[2];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int i = lane & 15;
+    const int g = lane >> 4;
+    const int sp = blockIdx.x;
+    const int f = blockIdx.y;
+    const int bt = blockIdx.z;
+    const int b0 = bt * 8;
+    const int A = a.A, F = a.F, T = a.T, B = a.B;
+    const int t_begin = sp * a.tsplit;
+    const int t_end = min(T, t_begin + a.tsplit);
+    const int ntiles = (t_end - t_begin + SG_TC - 1) / SG_TC;
+
+    uint32_t pre[PER];
+    auto load_tile = [&](int t0) {
+        const int tt = t0 + lane;
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int an = wave + ADJ_WAVES * k;
+            pre[k] = (an < A && tt < t_end) ? a.x[((size_t)an * F + f) * T + tt] : 0u;
+        }
+    };
+    auto store_tile = [&](uint32_t* dst) {
+#pragma unroll
+        for (int k = 0; k < PER; k++) dst[lane * XS + wave + ADJ_WAVES * k] = pre[k];
+    };
+    load_tile(t_begin);
+
+    // Phasor scale per pol: the largest component magnitude of this
+    // workgroup's beams (an exact maximum, so the order of the atomics does
+    // not matter).
+    if (tid < 2) s_wmax[tid] = 0u;
+    __syncthreads();
+    {
+        float mx[2] = {0.f, 0.f};
+        for (int e = tid; e < 2 * 8 * A; e += blockDim.x) {
+            const int p = e / (8 * A), r = e - p * 8 * A;
+            const int bb = r / A, an = r - bb * A, b = b0 + bb;
+            if (b < B) {
+                const float2 w = a.w[(((size_t)b * A + an) * F + f) * 2 + p];
+                mx[p] = fmaxf(mx[p], fmaxf(fabsf(w.x), fabsf(w.y)));
+            }
+        }
+        atomicMax(&s_wmax[0], __float_as_uint(mx[0]));
+        atomicMax(&s_wmax[1], __float_as_uint(mx[1]));
+    }
+    __syncthreads();
+    float wsc[2], wuns[2];
+    pow2_scale(__uint_as_float(s_wmax[0]), wsc[0], wuns[0]);
+    pow2_scale(__uint_as_float(s_wmax[1]), wsc[1], wuns[1]);
+    // Forward B operand: lane (g, i) of block blk holds k = 8g + 2q + c ->
+    // (antenna 16 blk + 4g + q, component c of x), column i -> (beam
+    // b0 + i/2, re/im).  y_r = x_r w_r - x_i w_i, y_i = x_r w_i + x_i w_r.
+    for (int e = tid; e < 2 * NAT * 64; e += blockDim.x) {
+        const int ln = e & 63, pb = e >> 6;
+        const int p = pb / NAT, blk = pb - p * NAT;
+        const int li = ln & 15, lg = ln >> 4;
+        const int b = b0 + (li >> 1), co = li & 1;
+        halfx8 hi, lo;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int an = blk * 16 + 4 * lg + q;
+            float2 w = make_float2(0.f, 0.f);
+            if (b < B && an < A) w = a.w[(((size_t)b * A + an) * F + f) * 2 + p];
+            const float v0 = (co == 0 ? w.x : w.y) * wsc[p];     // multiplies Re x
+            const float v1 = (co == 0 ? -w.y : w.x) * wsc[p];    // multiplies Im x
+            split_f16(v0, hi, lo, 2 * q);
+            split_f16(v1, hi, lo, 2 * q + 1);
+        }
+        s_wh[(pb * 2 + 0) * 64 + ln] = hi;
+        s_wh[(pb * 2 + 1) * 64 + ln] = lo;
+    }
+    store_tile(s_x);
+    if (ntiles > 1) load_tile(t_begin + SG_TC);
+    __syncthreads();
+
+    const int bcol = b0 + (i >> 1);
+    const float lam = bcol < B ? a.lambda[bcol] : 0.f;
+    const float swap_sign = (i & 1) ? -1.f : 1.f;
+    floatx4 acc2[2][NAT];
+#pragma unroll
+    for (int p = 0; p < 2; p++)
+#pragma unroll
+        for (int at = 0; at < NAT; at++) acc2[p][at] = floatx4{0.f, 0.f, 0.f, 0.f};
+    float loss = 0.f;
+    const floatx4 zero4 = {0.f, 0.f, 0.f, 0.f};
+
+    for (int k = 0; k < ntiles; k++) {
+        const uint32_t* sx = s_x + (k & 1) * SG_TC * XS;
+        const int trow = wave * 16;
+        // Forward: rows = this wave's 16 times; one 16-byte LDS read gives
+        // 4 antennas x both pols.  The phasor operands are re-read from LDS
+        // every tile (an opaque offset keeps the compiler from holding all
+        // 4 NAT of them in registers across the loop).
+        int woff = lane;
+        asm volatile("" : "+v"(woff));
+        const halfx8* wh = s_wh + woff;
+        floatx4 y[2] = {zero4, zero4};
+#pragma unroll
+        for (int blk = 0; blk < NAT; blk++) {
+            const uint4 d = *reinterpret_cast<const uint4*>(sx + (trow + i) * XS + blk * 16 + 4 * g);
+            halfx8 xa[2];
+            const uint32_t dw[4] = {d.x, d.y, d.z, d.w};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                halfx2 h0, h1;
+                ci8_to_half2x2(dw[q], h0, h1);
+                xa[0][2 * q] = h0[0]; xa[0][2 * q + 1] = h0[1];
+                xa[1][2 * q] = h1[0]; xa[1][2 * q + 1] = h1[1];
+            }
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                y[p] = mfma_f16(xa[p], wh[((p * NAT + blk) * 2 + 0) * 64], y[p]);
+                y[p] = mfma_f16(xa[p], wh[((p * NAT + blk) * 2 + 1) * 64], y[p]);
+            }
+        }
+        // gy = -lambda y (unscaled), its wave maximum, and the B operands.
+        halfx8 gh[2], gl[2];
+        float gsc_inv[2];
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            float gyo[4], gys[4], m = 0.f;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const float v = y[p][r] * wuns[p];
+                loss += lam * v * v;
+                gyo[r] = -lam * v;
+                gys[r] = swap_sign * swap_pairs(gyo[r]);
+                m = fmaxf(m, fabsf(gyo[r]));
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+            float sc;
+            pow2_scale(m, sc, gsc_inv[p]);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                split_f16(gyo[r] * sc, gh[p], gl[p], 2 * r);
+                split_f16(gys[r] * sc, gh[p], gl[p], 2 * r + 1);
+            }
+        }
+        // Adjoint: rows = antennas 16 at + i, K = times trow + 4g + q (x re, im).
+#pragma unroll
+        for (int at = 0; at < NAT; at++) {
+            halfx8 xa[2];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                halfx2 h0, h1;
+                ci8_to_half2x2(sx[(trow + 4 * g + q) * XS + at * 16 + i], h0, h1);
+                xa[0][2 * q] = h0[0]; xa[0][2 * q + 1] = h0[1];
+                xa[1][2 * q] = h1[0]; xa[1][2 * q + 1] = h1[1];
+            }
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                floatx4 t = mfma_f16(xa[p], gh[p], zero4);
+                t = mfma_f16(xa[p], gl[p], t);
+#pragma unroll
+                for (int r = 0; r < 4; r++) acc2[p][at][r] = fmaf(t[r], gsc_inv[p], acc2[p][at][r]);
+            }
+        }
+        if (k + 1 < ntiles) {
+            store_tile(s_x + ((k + 1) & 1) * SG_TC * XS);
+            if (k + 2 < ntiles) load_tile(t_begin + (k + 2) * SG_TC);
+        }
+        __syncthreads();
+    }
+
+    float* s_part = reinterpret_cast<float*>(lds_raw);
+#pragma unroll
+    for (int p = 0; p < 2; p++)
+#pragma unroll
+        for (int at = 0; at < NAT; at++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                s_part[((wave * 2 + p) * 16 + i) * APAD + at * 16 + 4 * g + r] = acc2[p][at][r];
+    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off, 64);
+    if (lane == 0) s_loss[wave] = loss;
+    __syncthreads();
+    constexpr int NPART = 2 * 16 * APAD;
+    float* out = a.partial + ((((size_t)sp * F + f) * 2) * a.btiles + bt) * 16 * APAD;
+    const size_t pstride = (size_t)a.btiles * 16 * APAD;
+    for (int e = tid; e < NPART; e += blockDim.x) {
+        const float v = ((s_part[e] + s_part[NPART + e]) + s_part[2 * NPART + e]) + s_part[3 * NPART + e];
+        const int p = e / (16 * APAD), r = e - p * 16 * APAD;
+        out[p * pstride + r] = v;
+    }
+    if (tid == 0)
+        a.loss_part[((size_t)sp * F + f) * a.btiles + bt] = ((s_loss[0] + s_loss[1]) + s_loss[2]) + s_loss[3];
+}
+
 // gw[b,a,f,p] = sum over time splits of the partial rows (b, re) and (b, im).
+// Threads follow the partial layout (antenna fastest), so the nsplit x 2
+// reads per output coalesce; the writes, one float2 per thread into
+// [B,A,F,1,P], are the strided side (8 B per thread, one pass).
 __global__ __launch_bounds__(256) void adjoint_reduce_kernel(AdjArgs a, int nsplit, float2* gw) {
-    const size_t n = (size_t)a.B * a.A * a.F * 2;
+    const size_t n = (size_t)a.F * 2 * a.btiles * 8 * a.A;
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n) return;
-    const int p = (int)(idx & 1);
-    size_t r = idx >> 1;
-    const int f = (int)(r % a.F);
-    r /= a.F;
+    size_t r = idx;
     const int an = (int)(r % a.A);
-    const int b = (int)(r / a.A);
-    const int bt = b >> 3, row = 2 * (b & 7);
+    r /= a.A;
+    const int bb = (int)(r % 8);
+    r /= 8;
+    const int bt = (int)(r % a.btiles);
+    r /= a.btiles;
+    const int p = (int)(r & 1);
+    const int f = (int)(r >> 1);
+    const int b = bt * 8 + bb;
+    if (b >= a.B) return;
     float re = 0.f, im = 0.f;
     for (int sp = 0; sp < nsplit; sp++) {
         const float* blk = a.partial + ((((size_t)sp * a.F + f) * 2 + p) * a.btiles + bt) * 16 * a.apad;
-        re += blk[row * a.apad + an];
-        im += blk[(row + 1) * a.apad + an];
+        re += blk[(2 * bb) * a.apad + an];
+        im += blk[(2 * bb + 1) * a.apad + an];
     }
-    gw[idx] = make_float2(re, im);   // [B,A,F,1,P] order: ((b*A + a)*F + f)*P + p
+    gw[(((size_t)b * a.A + an) * a.F + f) * 2 + p] = make_float2(re, im);
 }
 
 // loss = -sum of the workgroup partials, accumulated in FP64 in a fixed order.
@@ -392,7 +627,7 @@ __global__ __launch_bounds__(256) void adjoint_loss_kernel(const float* part, si
     if (threadIdx.x == 0) *loss = -s[0];
 }
 
-static int run_adjoint(bool fused, const void* input, const void* phasors, const void* grad_out,
+static int run_adjoint(bool fused, bool f16, const void* input, const void* phasors, const void* grad_out,
                        const float* beam_weights, double* loss, void* grad_phasors, uint64_t A,
                        uint64_t F, uint64_t T, uint64_t P, uint64_t B, hipStream_t s) {
     const char* who = fused ? "stokes-i gradient" : "beamformer adjoint";
@@ -419,15 +654,16 @@ static int run_adjoint(bool fused, const void* input, const void* phasors, const
     if (fused)
         for (uint64_t b = 0; b < B; b++) a.lambda[b] = beam_weights[b];
     // Time splits, each a whole number of tiles.  Plain adjoint: about 2048
-    // workgroups.  Fused: the split length that minimises rounds of
-    // 4 resident workgroups per CU x tiles per workgroup (longest on ties,
-    // which keeps the partial sums few).
+    // workgroups.  Fused: the split length that minimises rounds of resident
+    // workgroups (per CU: 4 FP32 or 3 split FP16 up to 64 or 48 antennas,
+    // else 2) x tiles per workgroup (longest on ties, which keeps the
+    // partial sums few).
     const int tc = fused ? SG_TC : ADJ_TC;
     const long tiles = (long)ceil_div(std::max<uint64_t>(T, 1), (uint64_t)tc);
     const long per_split = (long)F * a.btiles;
     long tiles_per;
     if (fused) {
-        const long slots = 256L * (a.n_at <= 4 ? 4 : 2);
+        const long slots = 256L * (f16 ? (a.n_at <= 3 ? 3 : 2) : (a.n_at <= 4 ? 4 : 2));
         long best = -1;
         tiles_per = tiles;
         for (long tp = tiles; tp >= 1; tp--) {
@@ -452,10 +688,22 @@ static int run_adjoint(bool fused, const void* input, const void* phasors, const
     a.partial = reinterpret_cast<float*>(ws);
     a.loss_part = a.partial + part_floats;
 
-    const size_t lds = fused ? ((size_t)2 * SG_TC * sg_xs(a.apad) + 2 * (size_t)a.nsteps * 64) * 4
-                             : ((size_t)ADJ_TC * adj_xs(a.apad) + ADJ_TC * ADJ_GYS + 2 * 16 * a.apad) * 4;
+    const size_t lds = !fused ? ((size_t)ADJ_TC * adj_xs(a.apad) + ADJ_TC * ADJ_GYS + 2 * 16 * a.apad) * 4
+                       : f16   ? (size_t)2 * SG_TC * sg_xs(a.apad) * 4 + (size_t)2 * a.n_at * 2 * 64 * 16
+                               : ((size_t)2 * SG_TC * sg_xs(a.apad) + 2 * (size_t)a.nsteps * 64) * 4;
     const void* fn = nullptr;
-    if (fused) {
+    if (fused && f16) {
+        switch (a.n_at) {
+            case 1: fn = reinterpret_cast<const void*>(&stokes_grad_f16_kernel<1>); break;
+            case 2: fn = reinterpret_cast<const void*>(&stokes_grad_f16_kernel<2>); break;
+            case 3: fn = reinterpret_cast<const void*>(&stokes_grad_f16_kernel<3>); break;
+            case 4: fn = reinterpret_cast<const void*>(&stokes_grad_f16_kernel<4>); break;
+            case 5: fn = reinterpret_cast<const void*>(&stokes_grad_f16_kernel<5>); break;
+            case 6: fn = reinterpret_cast<const void*>(&stokes_grad_f16_kernel<6>); break;
+            case 7: fn = reinterpret_cast<const void*>(&stokes_grad_f16_kernel<7>); break;
+            default: fn = reinterpret_cast<const void*>(&stokes_grad_f16_kernel<8>); break;
+        }
+    } else if (fused) {
         switch (a.n_at) {
             case 1: fn = reinterpret_cast<const void*>(&stokes_grad_kernel<1>); break;
             case 2: fn = reinterpret_cast<const void*>(&stokes_grad_kernel<2>); break;
@@ -477,7 +725,19 @@ static int run_adjoint(bool fused, const void* input, const void* phasors, const
         }
     }
     dim3 grid((unsigned)nsplit, (unsigned)F, (unsigned)a.btiles);
-    if (fused) {
+    const dim3 blk(64 * ADJ_WAVES);
+    if (fused && f16) {
+        switch (a.n_at) {
+            case 1: hipLaunchKernelGGL(stokes_grad_f16_kernel<1>, grid, blk, lds, s, a); break;
+            case 2: hipLaunchKernelGGL(stokes_grad_f16_kernel<2>, grid, blk, lds, s, a); break;
+            case 3: hipLaunchKernelGGL(stokes_grad_f16_kernel<3>, grid, blk, lds, s, a); break;
+            case 4: hipLaunchKernelGGL(stokes_grad_f16_kernel<4>, grid, blk, lds, s, a); break;
+            case 5: hipLaunchKernelGGL(stokes_grad_f16_kernel<5>, grid, blk, lds, s, a); break;
+            case 6: hipLaunchKernelGGL(stokes_grad_f16_kernel<6>, grid, blk, lds, s, a); break;
+            case 7: hipLaunchKernelGGL(stokes_grad_f16_kernel<7>, grid, blk, lds, s, a); break;
+            default: hipLaunchKernelGGL(stokes_grad_f16_kernel<8>, grid, blk, lds, s, a); break;
+        }
+    } else if (fused) {
         switch (a.n_at) {
             case 1: hipLaunchKernelGGL(stokes_grad_kernel<1>, grid, dim3(64 * ADJ_WAVES), lds, s, a); break;
             case 2: hipLaunchKernelGGL(stokes_grad_kernel<2>, grid, dim3(64 * ADJ_WAVES), lds, s, a); break;
@@ -491,7 +751,7 @@ static int run_adjoint(bool fused, const void* input, const void* phasors, const
     } else {
         hipLaunchKernelGGL(adjoint_kernel, grid, dim3(64 * ADJ_WAVES), lds, s, a);
     }
-    const size_t n = (size_t)B * A * F * 2;
+    const size_t n = (size_t)F * 2 * a.btiles * 8 * A;
     hipLaunchKernelGGL(adjoint_reduce_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, a,
                        (int)nsplit, static_cast<float2*>(grad_phasors));
     if (fused)
@@ -506,15 +766,17 @@ using namespace bl;
 extern "C" int blade_beamformer_adjoint(const void* input, const void* grad_output,
                                         void* grad_phasors, uint64_t A, uint64_t F, uint64_t T,
                                         uint64_t P, uint64_t B, blade_stream_t stream) {
-    return run_adjoint(false, input, nullptr, grad_output, nullptr, nullptr, grad_phasors, A, F, T,
+    return run_adjoint(false, false, input, nullptr, grad_output, nullptr, nullptr, grad_phasors, A, F, T,
                        P, B, reinterpret_cast<hipStream_t>(stream));
 }
 
-extern "C" int blade_beamformer_stokes_i_grad(const void* input, const void* phasors,
+extern "C" int blade_beamformer_stokes_i_grad(int precision, const void* input, const void* phasors,
                                               const float* beam_weights, double* loss,
                                               void* grad_phasors, uint64_t A, uint64_t F,
                                               uint64_t T, uint64_t P, uint64_t B,
                                               blade_stream_t stream) {
-    return run_adjoint(true, input, phasors, nullptr, beam_weights, loss, grad_phasors, A, F, T, P,
+    if (precision != BLADE_PREC_F32 && precision != BLADE_PREC_F16X2)
+        return fail("stokes-i gradient: unknown precision");
+    return run_adjoint(true, precision == BLADE_PREC_F16X2, input, phasors, nullptr, beam_weights, loss, grad_phasors, A, F, T, P,
                        B, reinterpret_cast<hipStream_t>(stream));
 }

@@ -57,24 +57,6 @@ __device__ __forceinline__ floatx4 mfma_f32(float a, float b, floatx4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ floatx4 mfma_f16(halfx8 a, halfx8 b, floatx4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
-}
-
-// Exact CI8 -> FP16: int8 b becomes the biased byte u = b + 128, placed in the
-// mantissa of 1024.0 (0x6400) to give 1024 + u exactly; subtracting 1152 gives b.
-// Both pols of a dword at once: v_perm_b32 places each biased byte below a
-// 0x64 byte (0x64uu = 1024 + u in FP16), one packed subtract removes the bias.
-// {src0, src1} is read as bytes 7..4 = src0, 3..0 = src1.
-__device__ __forceinline__ void ci8_to_half2x2(uint32_t d, halfx2& p0, halfx2& p1) {
-    const uint32_t u = d ^ 0x80808080u;
-    const uint32_t b0 = __builtin_amdgcn_perm(u, 0x64646464u, 0x00050004u);   // (u0, 64, u1, 64)
-    const uint32_t b1 = __builtin_amdgcn_perm(u, 0x64646464u, 0x00070006u);   // (u2, 64, u3, 64)
-    const halfx2 bias = {(_Float16)1152.0f, (_Float16)1152.0f};
-    p0 = __builtin_bit_cast(halfx2, b0) - bias;
-    p1 = __builtin_bit_cast(halfx2, b1) - bias;
-}
-
 __device__ __forceinline__ float ci8_byte(uint32_t d, int idx) {
     return (float)(int8_t)((d >> (8 * idx)) & 0xFFu);
 }

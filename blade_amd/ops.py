@@ -181,8 +181,11 @@ def beamformer_adjoint(x: torch.Tensor, grad_output: torch.Tensor, out: torch.Te
 
 
 def stokes_i_loss_and_grad(x: torch.Tensor, phasors: torch.Tensor, beam_weights,
-                           stream=None) -> tuple[torch.Tensor, torch.Tensor]:
-    """L = -sum_b beam_weights[b] sum |y_b|^2 (FP64 device scalar) and dL/dconj(w) CF32."""
+                           stream=None, precision: str = "f32") -> tuple[torch.Tensor, torch.Tensor]:
+    """L = -sum_b beam_weights[b] sum |y_b|^2 (FP64 device scalar) and dL/dconj(w) CF32.
+
+    ``precision``: "f32" (FP32 matrix cores) or "f16x2" (FP16 matrix cores,
+    hi + lo split phasors and gradients, FP32 accumulation)."""
     A, F, T, P = _shape4(x, "ci8")
     B = phasors.shape[0]
     if tuple(phasors.shape) != (B, A, F, 1, P) or phasors.dtype != torch.complex64:
@@ -193,7 +196,8 @@ def stokes_i_loss_and_grad(x: torch.Tensor, phasors: torch.Tensor, beam_weights,
     weights = (ct.c_float * B)(*lam)
     loss = torch.empty((), dtype=torch.float64, device=x.device)
     grad = torch.empty((B, A, F, 1, P), dtype=torch.complex64, device=x.device)
-    call("blade_beamformer_stokes_i_grad", _ptr(x), _ptr(phasors), ct.cast(weights, ct.c_void_p),
+    prec = {"f32": _lib.PREC_F32, "f16x2": _lib.PREC_F16X2}[precision]
+    call("blade_beamformer_stokes_i_grad", prec, _ptr(x), _ptr(phasors), ct.cast(weights, ct.c_void_p),
          _ptr(loss), _ptr(grad), A, F, T, P, B, stream_handle(stream))
     return loss, grad
 

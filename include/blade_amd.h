@@ -178,9 +178,11 @@ int blade_beamformer_adjoint(const void* input, const void* grad_output, void* g
  * loss: one FP64 value in device memory; grad_phasors = dL/dconj(w) CF32
  * [B,A,F,1,P] (PyTorch's gradient is twice this).  The beams are recomputed
  * tile by tile and never stored.  beam_weights: host array of B floats,
- * B <= 64; P = 2, A <= 128.
+ * B <= 64; P = 2, A <= 128.  precision: BLADE_PREC_F32 (FP32 matrix cores)
+ * or BLADE_PREC_F16X2 (FP16 matrix cores with hi + lo phasors and
+ * gradients, FP32 accumulation; error at the FP32 level).
  */
-int blade_beamformer_stokes_i_grad(const void* input, const void* phasors,
+int blade_beamformer_stokes_i_grad(int precision, const void* input, const void* phasors,
                                    const float* beam_weights, double* loss, void* grad_phasors,
                                    uint64_t A, uint64_t F, uint64_t T, uint64_t P, uint64_t B,
                                    blade_stream_t stream);

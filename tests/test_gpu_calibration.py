@@ -53,26 +53,28 @@ def test_adjoint_matches_oracle(A, F, T, B):
                                          (64, 2, 1000, 3, [0.2, 1, 3]), (80, 1, 130, 5, [1] * 5),
                                          (100, 1, 320, 20, [(-1) ** b * (b + 1) / 7 for b in range(20)]),
                                          (128, 1, 700, 8, [1, 0, 0, 0, 0, 0, 0, 1])])
-def test_stokes_i_grad_matches_oracle(A, F, T, B, lam):
+@pytest.mark.parametrize("precision", ["f32", "f16x2"])
+def test_stokes_i_grad_matches_oracle(A, F, T, B, lam, precision):
     from blade_amd import ops
     rng = np.random.default_rng(A + T)
     x = O.ci8_gaussian((A, F, T, 2), rng=rng)
     w = crand(rng, B, A, F, 1, 2)
-    loss, grad = ops.stokes_i_loss_and_grad(dev(x), dev(w), lam)
+    loss, grad = ops.stokes_i_loss_and_grad(dev(x), dev(w), lam, precision=precision)
     ref_loss, ref_grad = O.stokes_i_loss_and_grad_phasors(x, w, lam)
     assert abs(float(host(loss)) - ref_loss) / abs(ref_loss) < TOL
     assert O.norm_rel_err(host(grad), ref_grad) < TOL
 
 
-def test_stokes_i_grad_is_deterministic():
+@pytest.mark.parametrize("precision", ["f32", "f16x2"])
+def test_stokes_i_grad_is_deterministic(precision):
     # Partial sums are combined in a fixed order: repeated calls agree bit for bit.
     from blade_amd import ops
     rng = np.random.default_rng(11)
     x = dev(O.ci8_gaussian((42, 8, 2048, 2), rng=rng))
     w = dev(crand(rng, 4, 42, 8, 1, 2))
-    l0, g0 = ops.stokes_i_loss_and_grad(x, w, [1, 0, 0, 0])
+    l0, g0 = ops.stokes_i_loss_and_grad(x, w, [1, 0, 0, 0], precision=precision)
     for _ in range(3):
-        l1, g1 = ops.stokes_i_loss_and_grad(x, w, [1, 0, 0, 0])
+        l1, g1 = ops.stokes_i_loss_and_grad(x, w, [1, 0, 0, 0], precision=precision)
         assert float(host(l1)) == float(host(l0))
         assert np.array_equal(host(g1), host(g0))
 
@@ -176,11 +178,12 @@ def calibrator(A, F, T, seed, tau_max=100e-9, snr_db=0.0):
     return x, w.astype(np.complex64), g
 
 
-def test_adam_calibration_recovers_gain_phases():
+@pytest.mark.parametrize("precision", ["f32", "f16x2"])
+def test_adam_calibration_recovers_gain_phases(precision):
     from blade_amd.calibration import calibrate
     A, F, T = 12, 4, 512
     x, w, g_true = calibrator(A, F, T, seed=8)
-    gains, _, losses = calibrate(dev(x), dev(w), steps=200, lr=0.05)
+    gains, _, losses = calibrate(dev(x), dev(w), steps=200, lr=0.05, precision=precision)
     ge = host(gains)
     u = ge * g_true / np.abs(ge * g_true)
     coherence = np.abs(u.sum(axis=0)) / A                                   # [F, P]
