@@ -9,6 +9,7 @@ writes a buffer of the neighbouring cycle fails it.
 """
 
 import math
+import time
 
 import numpy as np
 import pytest
@@ -42,7 +43,11 @@ def drive(p, n_blocks, n_outputs, transfer_in, transfer_out):
             state["done"] += 1
         return bl.result.success
 
-    for _ in range(50 * n_blocks):               # bounded
+    # dequeue() polls without blocking once fewer blocks than streams are in
+    # flight (as the reference's does), so the loop is bounded by time, not
+    # by a count of polls.
+    deadline = time.monotonic() + 120.0
+    while time.monotonic() < deadline:
         if state["in"] == n_blocks and state["done"] == n_outputs:
             break
         if state["in"] < n_blocks:
