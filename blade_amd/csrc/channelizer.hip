@@ -352,8 +352,9 @@ __global__ __launch_bounds__(1024) void chan_big_pass1(const float2* __restrict_
         for (int e = threadIdx.x; e < L * N2; e += blockDim.x) {
             const int q = e / N2, k2 = e - q * N2;
             const int pp = q / cols, c = q - pp * cols;
-            const uint64_t n1 = (uint64_t)(n1_0 + c);
-            const float2 w = twC[(n1 * (uint64_t)k2) % (uint64_t)C];
+            // n1 k2 < N1 N2 = C: the exponent needs no reduction
+            const int n1 = n1_0 + c;
+            const float2 w = twC[n1 * k2];
             y[((sp * P + pp) * N1 + n1) * (uint64_t)N2 + k2] = cmul(data[q * stride + k2], w);
         }
         __syncthreads();
@@ -1400,7 +1401,7 @@ struct BigPlan {
 constexpr int kMaxBigRate = 1 << 20;     // the pass-1 twiddle table: 8 MiB
 
 static int big_cols(int n_other, int P, int len) {
-    int cols = 16;
+    int cols = 64;
     while (cols > 1 && (P * cols * (len + 1) > kMaxLdsComplex || n_other % cols)) cols /= 2;
     return P * cols * (len + 1) <= kMaxLdsComplex ? cols : 0;
 }
