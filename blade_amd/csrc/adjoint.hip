@@ -224,17 +224,24 @@ __global__ __launch_bounds__(256, (NAT <= 4 ? 4 : 2)) void stokes_grad_kernel(Ad
     // Voltage staging: thread (wave, lane) takes antennas wave + 4k at time
     // lane of the tile (one coalesced 256-byte row per wave and k).
     uint32_t pre[PER];
+    bool pre_t_ok = true;
+    // Unconditional loads from clamped addresses (no branch per antenna row);
+    // padded antennas and samples past the split are zeroed when the tile is
+    // written to LDS, so the loads stay in flight until then.
     auto load_tile = [&](int t0) {
         const int tt = t0 + lane;
+        pre_t_ok = tt < t_end;
+        const int ttc = min(tt, t_end - 1);
 #pragma unroll
-        for (int k = 0; k < PER; k++) {
-            const int an = wave + ADJ_WAVES * k;
-            pre[k] = (an < A && tt < t_end) ? a.x[((size_t)an * F + f) * T + tt] : 0u;
-        }
+        for (int k = 0; k < PER; k++)
+            pre[k] = a.x[((size_t)min(wave + ADJ_WAVES * k, A - 1) * F + f) * T + ttc];
     };
     auto store_tile = [&](uint32_t* dst) {
 #pragma unroll
-        for (int k = 0; k < PER; k++) dst[lane * XS + wave + ADJ_WAVES * k] = pre[k];
+        for (int k = 0; k < PER; k++) {
+            const int an = wave + ADJ_WAVES * k;
+            dst[lane * XS + an] = (pre_t_ok && an < A) ? pre[k] : 0u;
+        }
     };
     load_tile(t_begin);
 
@@ -286,14 +293,17 @@ __global__ __launch_bounds__(256, (NAT <= 4 ? 4 : 2)) void stokes_grad_kernel(Ad
         const int trow = wave * 16;
         // Forward: y^T rows = this wave's 16 times, both pols from one dword.
         floatx4 y[2] = {floatx4{0.f, 0.f, 0.f, 0.f}, floatx4{0.f, 0.f, 0.f, 0.f}};
-        for (int ag = 0; ag < n_ag; ag++) {
-            const uint32_t d = sx[(trow + i) * XS + ag * 4 + g];
 #pragma unroll
-            for (int p = 0; p < 2; p++) {
-                const float w0 = s_w[(p * nsteps + 2 * ag) * 64 + lane];
-                const float w1 = s_w[(p * nsteps + 2 * ag + 1) * 64 + lane];
-                y[p] = adj_mfma(ci8_component(d, 2 * p), w0, y[p]);
-                y[p] = adj_mfma(ci8_component(d, 2 * p + 1), w1, y[p]);
+        for (int ag = 0; ag < 4 * NAT; ag++) {
+            if (ag < n_ag) {                       // uniform
+                const uint32_t d = sx[(trow + i) * XS + ag * 4 + g];
+#pragma unroll
+                for (int p = 0; p < 2; p++) {
+                    const float w0 = s_w[(p * nsteps + 2 * ag) * 64 + lane];
+                    const float w1 = s_w[(p * nsteps + 2 * ag + 1) * 64 + lane];
+                    y[p] = adj_mfma(ci8_component(d, 2 * p), w0, y[p]);
+                    y[p] = adj_mfma(ci8_component(d, 2 * p + 1), w1, y[p]);
+                }
             }
         }
         // Lane (g, i), register r: y at time trow + 4g + r, column i.  Its
@@ -410,17 +420,24 @@ __global__ __launch_bounds__(256, (NAT <= 3 ? 3 : 2)) void stokes_grad_f16_kerne
     const int ntiles = (t_end - t_begin + SG_TC - 1) / SG_TC;
 
     uint32_t pre[PER];
+    bool pre_t_ok = true;
+    // Unconditional loads from clamped addresses (no branch per antenna row);
+    // padded antennas and samples past the split are zeroed when the tile is
+    // written to LDS, so the loads stay in flight until then.
     auto load_tile = [&](int t0) {
         const int tt = t0 + lane;
+        pre_t_ok = tt < t_end;
+        const int ttc = min(tt, t_end - 1);
 #pragma unroll
-        for (int k = 0; k < PER; k++) {
-            const int an = wave + ADJ_WAVES * k;
-            pre[k] = (an < A && tt < t_end) ? a.x[((size_t)an * F + f) * T + tt] : 0u;
-        }
+        for (int k = 0; k < PER; k++)
+            pre[k] = a.x[((size_t)min(wave + ADJ_WAVES * k, A - 1) * F + f) * T + ttc];
     };
     auto store_tile = [&](uint32_t* dst) {
 #pragma unroll
-        for (int k = 0; k < PER; k++) dst[lane * XS + wave + ADJ_WAVES * k] = pre[k];
+        for (int k = 0; k < PER; k++) {
+            const int an = wave + ADJ_WAVES * k;
+            dst[lane * XS + an] = (pre_t_ok && an < A) ? pre[k] : 0u;
+        }
     };
     load_tile(t_begin);
 
