@@ -88,7 +88,8 @@ def build(incremental: bool = False, jobs: int | None = None, verbose: bool = Tr
         bind_obj = fut_b.result()
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp)]
-         + [str(o) for o in objs] + ["-Wl,--no-undefined", "-lamdhip64", "-lhiprtc"])
+         + [str(o) for o in objs] + ["-Wl,--no-undefined", "-lamdhip64", "-lhiprtc",
+                                      "-lrocprofiler-sdk-roctx"])
     os.replace(tmp, LIB)
     # pybind11 extension (the Python mirror of the C++ API) over the library.
     ext = PKG / ("_blade_ext" + sysconfig.get_config_var("EXT_SUFFIX"))

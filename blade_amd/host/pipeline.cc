@@ -2,9 +2,24 @@
 // (reference src/pipeline.cc:7-141, src/runner.cc:17-68, src/memory/profiler.cc).
 #include "blade/base.hh"
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
+#include <cstdlib>
 #include <thread>
 
 namespace Blade {
+
+namespace {
+// BLADE_ROCTX=1: every module's process() call becomes a ROCTX range named
+// after the module ("rocprofv3 --marker-trace" shows them beside the kernels).
+bool markersEnabled() {
+    static const bool on = [] {
+        const char* v = std::getenv("BLADE_ROCTX");
+        return v && std::atoi(v) != 0;
+    }();
+    return on;
+}
+}  // namespace
 
 #undef BL_LOG_DOMAIN
 #define BL_LOG_DOMAIN "PIPELINE"
@@ -159,7 +174,9 @@ Result Pipeline::runModules(const U64& index) {
         const U64 localStepCount = (_computeStepCount / localStepOffset) % ratio;
         BL_TRACE("[M: '{}', R: {}]: Step Count: {}, Step Offset: {}, Step Ratio: {}", module->name(),
                  module->getComputeRatio(), localStepCount, localStepOffset, ratio);
+        if (markersEnabled()) roctxRangePushA(module->name().c_str());
         const Result result = module->process(localStepCount, _streams[index]);
+        if (markersEnabled()) roctxRangePop();
         if (result == Result::PIPELINE_EXHAUSTED) {
             BL_INFO("Module finished pipeline execution at {} lifetime compute cycles.",
                     _computeLifetimeCycles);
