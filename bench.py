@@ -565,7 +565,14 @@ def _drive_async(bl, p, n_outputs, ins, outs):
     while state["done"] < n_outputs:
         p.enqueue(input_callback, output_callback, state["in"], state["out"])
         p.dequeue(callback)
-    return time.perf_counter() - t0
+    dt = time.perf_counter() - t0
+    # Untimed: collect the block still in flight, so that the next run starts
+    # from an idle runner and none is left queued when the runner goes away.
+    deadline = time.perf_counter() + 60
+    while p.dequeue(callback) != bl.result.runner_queue_empty:
+        if time.perf_counter() > deadline:
+            raise RuntimeError("runner did not drain within 60 s")
+    return dt
 
 
 def _mode_b_pipeline(bl, config, in_shape, out_shape):
