@@ -21,7 +21,9 @@
 // so the forward accumulator a lane holds (times 4g..4g+3 of column i) is
 // exactly the adjoint's B operand for K = those four times: gy never passes
 // through LDS.  G's second component block (c' = imaginary part of x) is the
-// neighbouring column's value, one lane swap away.
+// neighbouring column's value, one lane swap away.  With at most 4 beams the
+// FP32 form runs stokes_grad_b4_kernel instead, on 4x4-block matrix
+// instructions whose columns are exactly the beams.
 //
 // Plain adjoint (adjoint_kernel, gy from memory): rows (beam, re/im),
 // K = (time, component), columns = antennas.
@@ -367,6 +369,233 @@ __global__ __launch_bounds__(256, (NAT <= 4 ? 4 : 2)) void stokes_grad_kernel(Ad
         a.loss_part[((size_t)sp * F + f) * a.btiles + bt] = ((s_loss[0] + s_loss[1]) + s_loss[2]) + s_loss[3];
 }
 
+// ---------------------------------------------------------------------------
+// Stokes-I loss and gradient for at most 4 beams (FP32), on
+// v_mfma_f32_4x4x1_16b_f32: 16 independent 4x4 blocks with K = 1.  Lane
+// 4 blk + c holds row c of block blk's A operand, column c of its B operand,
+// and column c of its result (register r = row r).  With 4 beams the columns
+// are exactly the beams, so no matrix work goes to absent beams (the 16x16
+// tiles of stokes_grad_kernel are half empty there).  Complex products take
+// three real products (P3 - P1 - P2 form).
+//   Forward, blocks = groups of 4 times, K = one antenna per instruction:
+//     lane (blk, c) feeds x[a][t0 + 4 blk + c] and w[a][beam c]; register r
+//     of the result is y at time t0 + 4 blk + r, beam c.
+//   Adjoint, blocks = groups of 4 antennas, K = one time per instruction:
+//     lane (blk, c) feeds conj(x)[4 blk + c][t] and gy[beam c][t]; register r
+//     is gw of antenna 4 blk + r, beam c.  gy passes through a per-wave LDS
+//     tile [p][product][beam][time], read as broadcasts (all blocks read the
+//     same four addresses).
+// Each wave takes 64-time tiles; partial sums are added across waves in LDS
+// in wave order and across time splits by adjoint_reduce_kernel, whose
+// partial layout this kernel writes (rows 2 b and 2 b + 1, beam tile 0).
+// Needs A <= 64, T % 4 == 0, A F T < 2^31 and 16-byte aligned voltages.
+// ---------------------------------------------------------------------------
+constexpr int SG4_TC = 64;            // times per wave tile
+constexpr int SG4_WS = 8;             // LDS floats per (antenna, beam): (wr, wi, wr + wi) per pol, padded
+// gy tile rows ([p][product][beam], SG4_TC times each) are padded by 4
+// floats, so the four beams' rows that a read instruction broadcasts start
+// on different LDS banks (unpadded, their 256-byte stride put all four on the
+// same banks: 7 conflict cycles per LDS instruction).
+constexpr int SG4_GS = SG4_TC + 4;
+constexpr int SG4_GY = 2 * 3 * 4 * SG4_GS;   // floats of one wave's gy tile
+constexpr int SG4_WGS = 3;            // resident workgroups per CU (launch bounds)
+
+__device__ __forceinline__ floatx4 mfma4(float a, float b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 0);
+}
+
+template <int NAT>
+__global__ __launch_bounds__(256, SG4_WGS) void stokes_grad_b4_kernel(AdjArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    constexpr int AMAX = 16 * NAT;
+    float* const s_w = reinterpret_cast<float*>(lds_raw);            // [A][4][SG4_WS]
+    float* const s_gy = s_w + (size_t)a.A * 4 * SG4_WS;               // [wave][SG4_GY]
+    __shared__ float s_loss[ADJ_WAVES];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int blk = lane >> 2;
+    const int c = lane & 3;
+    const int sp = blockIdx.x;
+    const int f = blockIdx.y;
+    const int A = a.A, F = a.F, T = a.T, B = a.B;
+    const int t_begin = sp * a.tsplit;
+    const int t_end = min(T, t_begin + a.tsplit);
+    const int ntiles = (t_end - t_begin + SG4_TC - 1) / SG4_TC;
+    const unsigned ant_stride = (unsigned)F * (unsigned)T;   // A F T < 2^31 (host check)
+
+    // Phasors of (antenna, beam): wr, wi, wr + wi of pol 0, then of pol 1.
+    for (int e = tid; e < A * 4; e += blockDim.x) {
+        const int an = e >> 2, b = e & 3;
+        float v[6];
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            const float2 w = b < B ? a.w[(((size_t)b * A + an) * F + f) * 2 + p] : make_float2(0.f, 0.f);
+            v[3 * p] = w.x;
+            v[3 * p + 1] = w.y;
+            v[3 * p + 2] = w.x + w.y;
+        }
+        float4* dst = reinterpret_cast<float4*>(s_w + (size_t)e * SG4_WS);
+        dst[0] = make_float4(v[0], v[1], v[2], v[3]);
+        dst[1] = make_float4(v[4], v[5], 0.f, 0.f);
+    }
+    __syncthreads();
+
+    const float lam = c < B ? a.lambda[c] : 0.f;
+    // Adjoint sums per pol: Q1 = sum gr xr, Q2 = sum gi xi, Q3 = sum (gr + gi)(xr - xi);
+    // gw = (Q1 + Q2) + j (Q3 - Q1 + Q2).
+    floatx4 q[2][3];
+#pragma unroll
+    for (int p = 0; p < 2; p++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) q[p][k] = floatx4{0.f, 0.f, 0.f, 0.f};
+    float loss = 0.f;
+    float* const gyw = s_gy + wave * SG4_GY;
+    // This lane's adjoint antenna row (clamped: rows past A are never stored).
+    const uint4* const xa = reinterpret_cast<const uint4*>(a.x + min(4 * blk + c, A - 1) * ant_stride + f * T);
+
+    // Forward voltages of tile kk: this lane's time, every antenna.  The
+    // stride and count are opaque to the compiler, which would otherwise
+    // keep one offset and one guard mask per antenna in scalar registers
+    // across tiles (and spill them).
+    uint32_t xd[AMAX];
+    auto load_fwd = [&](int kk) {
+        const int tl = t_begin + kk * SG4_TC + 4 * blk + c;
+        const uint32_t* const xf = a.x + (f * T + min(tl, t_end - 1));
+        unsigned st = ant_stride;
+        int na = A;
+        asm volatile("" : "+s"(st), "+s"(na));
+#pragma unroll
+        for (int an = 0; an < AMAX; an++)
+            if (an < na) xd[an] = xf[an * st];                    // uniform
+    };
+    // Adjoint voltages of 16 times from t: this lane's antenna row (times
+    // past tn re-read the last 4 in bounds; their gy is not used).
+    auto load_adj = [&](int t0, int tn, int t, uint4 (&xv)[4]) {
+#pragma unroll
+        for (int s4 = 0; s4 < 4; s4++) xv[s4] = xa[(t0 + min(t + 4 * s4, tn - 4)) >> 2];
+    };
+
+    for (int k = wave; k < ntiles; k += ADJ_WAVES) {
+        const int t0 = t_begin + k * SG4_TC;
+        const int tn = min(SG4_TC, t_end - t0);            // a multiple of 4
+        load_fwd(k);
+        uint4 xv[2][4];
+        load_adj(t0, tn, 0, xv[0]);                        // in flight during the forward
+        // ---- forward: this lane feeds time t0 + 4 blk + c (clamped to the
+        // split; results for times past it are zeroed below).  The antenna
+        // count is opaque here too: 48 hoisted guard masks would not fit in
+        // scalar registers.
+        int na = A;
+        asm volatile("" : "+s"(na));
+        floatx4 y[2][3];
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int m = 0; m < 3; m++) y[p][m] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int an = 0; an < AMAX; an++) {
+            if (an < na) {                                        // uniform
+                const uint32_t d = xd[an];
+                const float* wv = s_w + (an * 4 + c) * SG4_WS;
+                const float4 w0 = *reinterpret_cast<const float4*>(wv);
+                const float2 w1 = *reinterpret_cast<const float2*>(wv + 4);
+                const float wp[2][3] = {{w0.x, w0.y, w0.z}, {w0.w, w1.x, w1.y}};
+#pragma unroll
+                for (int p = 0; p < 2; p++) {
+                    const float xr = ci8_component(d, 2 * p), xi = ci8_component(d, 2 * p + 1);
+                    y[p][0] = mfma4(xr, wp[p][0], y[p][0]);
+                    y[p][1] = mfma4(xi, wp[p][1], y[p][1]);
+                    y[p][2] = mfma4(xr + xi, wp[p][2], y[p][2]);
+                }
+            }
+        }
+        // ---- gy = -lambda y at (time t0 + 4 blk + r, beam c): gr, gi, gr + gi
+        // into this wave's LDS tile [p][product][beam][time].  Times past the
+        // split read clamped voltages: their y, and so gy and the loss, are
+        // set to 0.
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            float g[3][4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const bool ok = 4 * blk + r < tn;
+                const float yr = ok ? y[p][0][r] - y[p][1][r] : 0.f;
+                const float yi = ok ? (y[p][2][r] - y[p][0][r]) - y[p][1][r] : 0.f;
+                loss += lam * (yr * yr + yi * yi);
+                const float gr = -lam * yr, gi = -lam * yi;
+                g[0][r] = gr;
+                g[1][r] = gi;
+                g[2][r] = gr + gi;
+            }
+#pragma unroll
+            for (int m = 0; m < 3; m++)
+                *reinterpret_cast<float4*>(gyw + ((p * 3 + m) * 4 + c) * SG4_GS + 4 * blk) =
+                    make_float4(g[m][0], g[m][1], g[m][2], g[m][3]);
+        }
+        // ---- adjoint: this lane feeds antenna 4 blk + c and gy of beam c;
+        // one instruction per (time, pol, product).  The wave reads back its
+        // own LDS writes (LDS operations of a wave complete in order).  The
+        // next 16 times' voltages load while these are computed.
+#pragma unroll
+        for (int h = 0; h < SG4_TC / 16; h++) {
+            if (h + 1 < SG4_TC / 16) load_adj(t0, tn, 16 * (h + 1), xv[(h + 1) & 1]);
+            if (16 * h < tn) {                                    // uniform
+#pragma unroll
+                for (int s4 = 0; s4 < 4; s4++) {
+                    const int tt = 16 * h + 4 * s4;
+                    if (tt < tn) {                                // uniform
+                        float4 gv[2][3];
+#pragma unroll
+                        for (int p = 0; p < 2; p++)
+#pragma unroll
+                            for (int m = 0; m < 3; m++)
+                                gv[p][m] = *reinterpret_cast<const float4*>(gyw + ((p * 3 + m) * 4 + c) * SG4_GS + tt);
+                        const uint4 xq = xv[h & 1][s4];
+                        const uint32_t dw[4] = {xq.x, xq.y, xq.z, xq.w};
+#pragma unroll
+                        for (int s = 0; s < 4; s++) {
+#pragma unroll
+                            for (int p = 0; p < 2; p++) {
+                                const float xr = ci8_component(dw[s], 2 * p), xi = ci8_component(dw[s], 2 * p + 1);
+                                q[p][0] = mfma4(xr, gv[p][0][s], q[p][0]);
+                                q[p][1] = mfma4(xi, gv[p][1][s], q[p][1]);
+                                q[p][2] = mfma4(xr - xi, gv[p][2][s], q[p][2]);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+
+    // This wave's sums into its own gy tile (its last reads of it are done):
+    // [p][row = 2 beam + re/im][antenna 64], then summed over waves in order.
+    float* const pw = gyw;
+#pragma unroll
+    for (int p = 0; p < 2; p++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int an = 4 * blk + r;
+            pw[(p * 8 + 2 * c) * 64 + an] = q[p][0][r] + q[p][1][r];
+            pw[(p * 8 + 2 * c + 1) * 64 + an] = (q[p][2][r] - q[p][0][r]) + q[p][1][r];
+        }
+    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off, 64);
+    if (lane == 0) s_loss[wave] = loss;
+    __syncthreads();
+    const int apad = a.apad;
+    float* out = a.partial + ((size_t)sp * F + f) * 2 * a.btiles * 16 * apad;
+    const size_t pstride = (size_t)a.btiles * 16 * apad;   // between pols
+    for (int e = tid; e < 2 * 8 * A; e += blockDim.x) {
+        const int an = e % A, pr = e / A;                   // pr = p * 8 + row
+        const float* src = s_gy + pr * 64 + an;
+        const float v = ((src[0] + src[SG4_GY]) + src[2 * SG4_GY]) + src[3 * SG4_GY];
+        out[(pr >> 3) * pstride + (pr & 7) * apad + an] = v;
+    }
+    if (tid == 0) a.loss_part[((size_t)sp * F + f) * a.btiles] = ((s_loss[0] + s_loss[1]) + s_loss[2]) + s_loss[3];
+}
+
 // Power-of-two scale s with m * s in [0.5, 1) for m > 0 (1 for m = 0), and 1/s.
 __device__ __forceinline__ void pow2_scale(float m, float& s, float& inv) {
     int ex = 0;
@@ -670,22 +899,26 @@ static int run_adjoint(bool fused, bool f16, const void* input, const void* phas
     a.btiles = (int)ceil_div(B, 8);
     if (fused)
         for (uint64_t b = 0; b < B; b++) a.lambda[b] = beam_weights[b];
+    // FP32 loss + gradient with at most 4 beams: the 4x4-block kernel.
+    const bool b4 = fused && !f16 && B <= 4 && A <= 64 && T % 4 == 0 && (uintptr_t)input % 16 == 0 &&
+                    A * F * T < (1ull << 31);
     // Time splits, each a whole number of tiles.  Plain adjoint: about 2048
     // workgroups.  Fused: the split length that minimises rounds of resident
     // workgroups (per CU: 4 FP32 or 3 split FP16 up to 64 or 48 antennas,
-    // else 2) x tiles per workgroup (longest on ties, which keeps the
-    // partial sums few).
-    const int tc = fused ? SG_TC : ADJ_TC;
+    // else 2; the 4x4-block kernel SG4_WGS) x tiles per wave (longest on
+    // ties, which keeps the partial sums few).
+    const int tc = b4 ? SG4_TC : fused ? SG_TC : ADJ_TC;
     const long tiles = (long)ceil_div(std::max<uint64_t>(T, 1), (uint64_t)tc);
     const long per_split = (long)F * a.btiles;
     long tiles_per;
     if (fused) {
-        const long slots = 256L * (f16 ? (a.n_at <= 3 ? 3 : 2) : (a.n_at <= 4 ? 4 : 2));
+        const long slots = 256L * (b4 ? SG4_WGS : f16 ? (a.n_at <= 3 ? 3 : 2) : (a.n_at <= 4 ? 4 : 2));
         long best = -1;
         tiles_per = tiles;
         for (long tp = tiles; tp >= 1; tp--) {
             const long ns = (tiles + tp - 1) / tp;
-            const long cost = ((ns * per_split + slots - 1) / slots) * tp;
+            const long per_wave = b4 ? (tp + ADJ_WAVES - 1) / ADJ_WAVES : tp;   // b4: waves take tiles in turn
+            const long cost = ((ns * per_split + slots - 1) / slots) * per_wave;
             if (best < 0 || cost < best) { best = cost; tiles_per = tp; }
         }
     } else {
@@ -706,10 +939,18 @@ static int run_adjoint(bool fused, bool f16, const void* input, const void* phas
     a.loss_part = a.partial + part_floats;
 
     const size_t lds = !fused ? ((size_t)ADJ_TC * adj_xs(a.apad) + ADJ_TC * ADJ_GYS + 2 * 16 * a.apad) * 4
+                       : b4    ? ((size_t)a.A * 4 * SG4_WS + (size_t)ADJ_WAVES * SG4_GY) * 4
                        : f16   ? (size_t)2 * SG_TC * sg_xs(a.apad) * 4 + (size_t)2 * a.n_at * 2 * 64 * 16
                                : ((size_t)2 * SG_TC * sg_xs(a.apad) + 2 * (size_t)a.nsteps * 64) * 4;
     const void* fn = nullptr;
-    if (fused && f16) {
+    if (b4) {
+        switch (a.n_at) {
+            case 1: fn = reinterpret_cast<const void*>(&stokes_grad_b4_kernel<1>); break;
+            case 2: fn = reinterpret_cast<const void*>(&stokes_grad_b4_kernel<2>); break;
+            case 3: fn = reinterpret_cast<const void*>(&stokes_grad_b4_kernel<3>); break;
+            default: fn = reinterpret_cast<const void*>(&stokes_grad_b4_kernel<4>); break;
+        }
+    } else if (fused && f16) {
         switch (a.n_at) {
             case 1: fn = reinterpret_cast<const void*>(&stokes_grad_f16_kernel<1>); break;
             case 2: fn = reinterpret_cast<const void*>(&stokes_grad_f16_kernel<2>); break;
@@ -743,7 +984,14 @@ static int run_adjoint(bool fused, bool f16, const void* input, const void* phas
     }
     dim3 grid((unsigned)nsplit, (unsigned)F, (unsigned)a.btiles);
     const dim3 blk(64 * ADJ_WAVES);
-    if (fused && f16) {
+    if (b4) {
+        switch (a.n_at) {
+            case 1: hipLaunchKernelGGL(stokes_grad_b4_kernel<1>, grid, blk, lds, s, a); break;
+            case 2: hipLaunchKernelGGL(stokes_grad_b4_kernel<2>, grid, blk, lds, s, a); break;
+            case 3: hipLaunchKernelGGL(stokes_grad_b4_kernel<3>, grid, blk, lds, s, a); break;
+            default: hipLaunchKernelGGL(stokes_grad_b4_kernel<4>, grid, blk, lds, s, a); break;
+        }
+    } else if (fused && f16) {
         switch (a.n_at) {
             case 1: hipLaunchKernelGGL(stokes_grad_f16_kernel<1>, grid, blk, lds, s, a); break;
             case 2: hipLaunchKernelGGL(stokes_grad_f16_kernel<2>, grid, blk, lds, s, a); break;

@@ -52,7 +52,11 @@ def test_adjoint_matches_oracle(A, F, T, B):
                                          (4, 1, 64, 1, [1]), (17, 3, 4100, 2, [1, -0.5]),
                                          (64, 2, 1000, 3, [0.2, 1, 3]), (80, 1, 130, 5, [1] * 5),
                                          (100, 1, 320, 20, [(-1) ** b * (b + 1) / 7 for b in range(20)]),
-                                         (128, 1, 700, 8, [1, 0, 0, 0, 0, 0, 0, 1])])
+                                         (128, 1, 700, 8, [1, 0, 0, 0, 0, 0, 0, 1]),
+                                         # at most 4 beams (FP32: the 4x4-block kernel), all weighted;
+                                         # and its fallbacks: A > 64, T not a multiple of 4
+                                         (33, 5, 3072, 4, [0.5, -1, 2, 1]), (65, 2, 256, 4, [1, 0.5, 0.25, 2]),
+                                         (10, 2, 130, 2, [1, 1])])
 @pytest.mark.parametrize("precision", ["f32", "f16x2"])
 def test_stokes_i_grad_matches_oracle(A, F, T, B, lam, precision):
     from blade_amd import ops
