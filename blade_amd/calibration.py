@@ -9,6 +9,9 @@ C-ABI (``blade_beamformer_adjoint``, ``blade_beamformer_stokes_i_grad``):
   detected (Stokes-I) power of weighted beams.  Forward and gradient come from
   one fused kernel that recomputes the beams tile by tile instead of storing
   them over time; backward only scales the saved gradient.
+* :class:`SteeredStokesIPower` -- the same loss of ``w * g/|g|`` as a
+  function of the gains ``g``: the steering and the chain rule through it
+  are two more kernels (``blade_steer_gains``, ``blade_gain_gradient``).
 
 Complex gradients follow PyTorch's convention (``dL/dRe w + j dL/dIm w``,
 i.e. twice the conjugate Wirtinger derivative), so the functions compose with
@@ -29,8 +32,8 @@ import torch
 
 from . import ops
 
-__all__ = ["Beamform", "StokesIPower", "beamform", "stokes_i_power", "calibrate",
-           "steer_phasors"]
+__all__ = ["Beamform", "StokesIPower", "SteeredStokesIPower", "beamform", "stokes_i_power",
+           "calibrate", "steer_phasors"]
 
 
 class Beamform(torch.autograd.Function):
@@ -66,6 +69,28 @@ class StokesIPower(torch.autograd.Function):
         (grad,) = ctx.saved_tensors
         # PyTorch's complex gradient is twice dL/dconj(w).
         return None, grad * (2.0 * grad_loss.to(torch.float32)), None, None
+
+
+class SteeredStokesIPower(torch.autograd.Function):
+    """``StokesIPower`` of ``w * g/|g|`` as a function of the gains ``g`` [A,F,P]
+    (``w`` is a constant).  One call runs the steering, the fused loss +
+    phasor gradient and the gain gradient; backward only scales the saved
+    gain gradient."""
+
+    @staticmethod
+    def forward(ctx, x, w, gains, beam_weights, precision="f32"):
+        w = w.contiguous()
+        gains = gains.detach().contiguous()
+        w_eff = ops.steer_gains(w, gains)
+        loss, grad_w = ops.stokes_i_loss_and_grad(x, w_eff, beam_weights, precision=precision)
+        ctx.save_for_backward(ops.gain_gradient(w, grad_w, gains))
+        return loss
+
+    @staticmethod
+    def backward(ctx, grad_loss):
+        (grad_g,) = ctx.saved_tensors
+        # PyTorch's complex gradient is twice dL/dconj(g).
+        return None, None, grad_g * (2.0 * grad_loss.to(torch.float32)), None, None
 
 
 def beamform(x: torch.Tensor, w: torch.Tensor) -> torch.Tensor:
@@ -126,8 +151,14 @@ def calibrate(x: torch.Tensor, w: torch.Tensor, steps: int = 200, lr: float = 0.
         ref = abs(float(stokes_i_power(x, steer_phasors(w, gains, delays, freqs_hz), beam_weights, precision)))
     inv = 1.0 / ref if ref > 0 else 1.0
 
+    bw = list(beam_weights) if beam_weights is not None else [1.0] + [0.0] * (w.shape[0] - 1)
+
     def step():
-        loss = stokes_i_power(x, steer_phasors(w, gains, delays, freqs_hz), beam_weights, precision)
+        if gains is not None and delays is None:
+            # gains only: steering and its chain rule on the device kernels
+            loss = SteeredStokesIPower.apply(x, w, gains, bw, precision)
+        else:
+            loss = stokes_i_power(x, steer_phasors(w, gains, delays, freqs_hz), beam_weights, precision)
         (loss * inv).backward()
         opt.step()
         return loss.detach() * inv

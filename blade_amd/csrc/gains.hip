@@ -1,0 +1,102 @@
+// Per-antenna gain steering and its gradient, for gain calibration on top of
+// the fused Stokes-I loss + phasor gradient (adjoint.hip).  No reference
+// counterpart: the reference's Phasor module applies fixed calibration
+// coefficients (src/modules/phasor/ata.cc:188-204, the `cal` factor); these
+// kernels let that factor be learned as unit-modulus gains u = g / |g|.
+//
+//   steer:    w_eff[b,a,f,0,p] = w[b,a,f,0,p] * u[a,f,p]
+//   gradient: given gw = dL/dconj(w_eff) for a real loss L,
+//             G = dL/dconj(u) = sum_b conj(w) gw, and through u = g / |g|
+//             (dconj(u)/dconj(g) = 1 / (2|g|), du/dconj(g) = -u^2 / (2|g|))
+//             dL/dconj(g) = (G - u^2 conj(G)) / (2 |g|),
+//             tangent to the unit circle.  |g| is floored at 1e-30 as in
+//             g / |g|.clamp_min(1e-30); below the floor u = g * 1e30 and
+//             dL/dconj(g) = G * 1e30.
+// Both are HBM streams over B*A*F*P complex values (one thread per output).
+#include "common.hh"
+
+namespace bl {
+
+constexpr float kGainFloor = 1e-30f;
+
+__device__ __forceinline__ float2 unit_gain(float2 g, float& n) {
+    n = sqrtf(g.x * g.x + g.y * g.y);
+    const float d = 1.f / fmaxf(n, kGainFloor);
+    return make_float2(g.x * d, g.y * d);
+}
+
+__global__ __launch_bounds__(256) void steer_gains_kernel(const float2* __restrict__ w, const float2* __restrict__ g,
+                                                         float2* __restrict__ out, uint64_t n, uint64_t afp) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float norm;
+    const float2 u = unit_gain(g[i % afp], norm);
+    out[i] = cmul(w[i], u);
+}
+
+__global__ __launch_bounds__(256) void gain_gradient_kernel(const float2* __restrict__ w, const float2* __restrict__ gw,
+                                                           const float2* __restrict__ g, float2* __restrict__ gg,
+                                                           uint64_t afp, int B) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= afp) return;
+    // G = sum_b conj(w) gw, beams in order.
+    float2 G = make_float2(0.f, 0.f);
+    for (int b = 0; b < B; b++) {
+        const float2 wv = w[b * afp + i], gv = gw[b * afp + i];
+        G.x += wv.x * gv.x + wv.y * gv.y;
+        G.y += wv.x * gv.y - wv.y * gv.x;
+    }
+    float norm;
+    const float2 u = unit_gain(g[i], norm);
+    float2 r;
+    if (norm >= kGainFloor) {
+        // (G - u^2 conj(G)) / (2 |g|)
+        const float2 u2 = cmul(u, u);
+        const float2 t = cmul(u2, make_float2(G.x, -G.y));
+        const float s = 0.5f / norm;
+        r = make_float2((G.x - t.x) * s, (G.y - t.y) * s);
+    } else {
+        r = make_float2(G.x / kGainFloor, G.y / kGainFloor);
+    }
+    gg[i] = r;
+}
+
+static int check_gain_args(const char* who, const void* a, const void* b, const void* c, uint64_t B, uint64_t A,
+                           uint64_t F, uint64_t P) {
+    if (!a || !b || !c) return fail(std::string(who) + ": null pointer");
+    if ((uintptr_t)a % 8 || (uintptr_t)b % 8 || (uintptr_t)c % 8) return fail(std::string(who) + ": misaligned buffer");
+    if (P != 2) return fail(std::string(who) + ": number of polarizations must be 2");
+    if (B > 4096 || A > 4096 || F > (1u << 24)) return fail(std::string(who) + ": shape too large");
+    return BLADE_SUCCESS;
+}
+
+}  // namespace bl
+
+using namespace bl;
+
+extern "C" int blade_steer_gains(const void* phasors, const void* gains, void* output, uint64_t B, uint64_t A,
+                                 uint64_t F, uint64_t P, blade_stream_t stream) {
+    int rc = check_gain_args("gain steering", phasors, gains, output, B, A, F, P);
+    if (rc) return rc;
+    const uint64_t afp = A * F * P, n = B * afp;
+    if (n == 0) return BLADE_SUCCESS;
+    hipLaunchKernelGGL(steer_gains_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), static_cast<const float2*>(phasors),
+                       static_cast<const float2*>(gains), static_cast<float2*>(output), n, afp);
+    return check_launch("gain steering");
+}
+
+extern "C" int blade_gain_gradient(const void* phasors, const void* grad_phasors, const void* gains,
+                                   void* grad_gains, uint64_t B, uint64_t A, uint64_t F, uint64_t P,
+                                   blade_stream_t stream) {
+    int rc = check_gain_args("gain gradient", phasors, grad_phasors, gains, B, A, F, P);
+    if (!rc) rc = check_gain_args("gain gradient", grad_gains, gains, gains, B, A, F, P);
+    if (rc) return rc;
+    const uint64_t afp = A * F * P;
+    if (afp == 0) return BLADE_SUCCESS;
+    hipLaunchKernelGGL(gain_gradient_kernel, dim3((unsigned)ceil_div(afp, 256)), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), static_cast<const float2*>(phasors),
+                       static_cast<const float2*>(grad_phasors), static_cast<const float2*>(gains),
+                       static_cast<float2*>(grad_gains), afp, (int)B);
+    return check_launch("gain gradient");
+}

@@ -202,6 +202,38 @@ def stokes_i_loss_and_grad(x: torch.Tensor, phasors: torch.Tensor, beam_weights,
     return loss, grad
 
 
+def _gain_shapes(phasors: torch.Tensor, gains: torch.Tensor):
+    if phasors.dim() != 5 or phasors.shape[3] != 1 or phasors.dtype != torch.complex64:
+        raise ValueError(f"phasors must be CF32 [B,A,F,1,P], got {tuple(phasors.shape)} {phasors.dtype}")
+    B, A, F, _, P = phasors.shape
+    if tuple(gains.shape) != (A, F, P) or gains.dtype != torch.complex64:
+        raise ValueError(f"gains must be CF32 [{A},{F},{P}], got {tuple(gains.shape)} {gains.dtype}")
+    if not (phasors.is_contiguous() and gains.is_contiguous()):
+        raise ValueError("phasors and gains must be contiguous")
+    return B, A, F, P
+
+
+def steer_gains(phasors: torch.Tensor, gains: torch.Tensor, stream=None) -> torch.Tensor:
+    """phasors * gains / |gains| (unit-modulus gains [A,F,P] on phasors [B,A,F,1,P])."""
+    B, A, F, P = _gain_shapes(phasors, gains)
+    out = torch.empty_like(phasors)
+    call("blade_steer_gains", _ptr(phasors), _ptr(gains), _ptr(out), B, A, F, P, stream_handle(stream))
+    return out
+
+
+def gain_gradient(phasors: torch.Tensor, grad_phasors: torch.Tensor, gains: torch.Tensor,
+                  stream=None) -> torch.Tensor:
+    """dL/dconj(gains) from grad_phasors = dL/dconj(phasors * gains / |gains|)."""
+    B, A, F, P = _gain_shapes(phasors, gains)
+    if tuple(grad_phasors.shape) != tuple(phasors.shape) or grad_phasors.dtype != torch.complex64 \
+            or not grad_phasors.is_contiguous():
+        raise ValueError("grad_phasors must be contiguous CF32 of the phasors' shape")
+    out = torch.empty_like(gains)
+    call("blade_gain_gradient", _ptr(phasors), _ptr(grad_phasors), _ptr(gains), _ptr(out), B, A, F, P,
+         stream_handle(stream))
+    return out
+
+
 def channelizer(x: torch.Tensor, rate: int, out: torch.Tensor | None = None, stream=None):
     """FFT channelizer (Modules::Channelizer), CF32 [A,F,T,P] -> [A,F*C,T/C,P]."""
     A, F, T, P = _shape4(x, "cf32")

@@ -188,6 +188,28 @@ int blade_beamformer_stokes_i_grad(int precision, const void* input, const void*
                                    blade_stream_t stream);
 
 /*
+ * Gain steering for calibration: output[b,a,f,0,p] = phasors[b,a,f,0,p] * u[a,f,p],
+ * u = gains / |gains| (|gains| floored at 1e-30).  CF32 phasors and output
+ * [B,A,F,1,P], CF32 gains [A,F,P]; P = 2.
+ * No reference counterpart: the reference's Phasor applies fixed calibration
+ * coefficients (src/modules/phasor/ata.cc:188-204); this makes them unit-modulus
+ * gains that calibration can learn.  For gain calibration callers.
+ */
+int blade_steer_gains(const void* phasors, const void* gains, void* output, uint64_t B, uint64_t A,
+                      uint64_t F, uint64_t P, blade_stream_t stream);
+
+/*
+ * Gradient through the gain steering: given grad_phasors = dL/dconj(w_eff) of a
+ * real loss for w_eff = phasors * gains / |gains| (as blade_beamformer_stokes_i_grad
+ * returns it), grad_gains[a,f,p] = dL/dconj(g) = (G - u^2 conj(G)) / (2|g|) with
+ * G = sum_b conj(phasors) grad_phasors and u = g / |g|.  Shapes as blade_steer_gains.
+ * No reference counterpart (see blade_steer_gains).  For gain calibration callers.
+ */
+int blade_gain_gradient(const void* phasors, const void* grad_phasors, const void* gains,
+                        void* grad_gains, uint64_t B, uint64_t A, uint64_t F, uint64_t P,
+                        blade_stream_t stream);
+
+/*
  * Channelizer — fftshift(FFT_C) of every C-sample chunk along T.
  * Replaces Modules::Channelizer<CF32,CF32>::process (src/modules/channelizer/base.cc:122-146,
  * pre-kernel channelizer.cu:5-19 and cuFFT C2C forward) with the general-rate

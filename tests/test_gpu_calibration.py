@@ -166,6 +166,75 @@ def test_gain_and_delay_gradients_through_torch():
     assert abs(float(host(d.grad)[0]) - float(dr.grad[0])) / float(dr.grad.abs().max()) < 1e-4
 
 
+def test_steer_gains_kernel():
+    from blade_amd import ops
+    rng = np.random.default_rng(21)
+    w0 = crand(rng, 3, 5, 4, 1, 2)
+    g0 = crand(rng, 5, 4, 2)
+    g0[1, 2, 0] = 0                                      # |g| below the floor: u = g * 1e30 = 0
+    got = host(ops.steer_gains(dev(w0), dev(g0)))
+    gd = g0.astype(np.complex128)
+    u = gd / np.maximum(np.abs(gd), 1e-30)
+    ref = w0.astype(np.complex128) * u[None, :, :, None, :]
+    assert O.norm_rel_err(got, ref) < 1e-6
+    with pytest.raises(ValueError, match="gains"):
+        ops.steer_gains(dev(w0), dev(g0[:4]))
+
+
+@pytest.mark.parametrize("B,lam,precision", [(2, [1.0, 0.5], "f32"), (2, [1.0, 0.5], "f16x2"),
+                                             (6, [1.0, 0, 0.5, 0, 0, -0.25], "f32")])
+def test_gain_gradient_finite_differences_fp64(B, lam, precision):
+    # The gain gradient of SteeredStokesIPower against central differences of
+    # the complex128 oracle loss in Re g and Im g (FP64, step 1e-6).
+    import torch
+    from blade_amd.calibration import SteeredStokesIPower
+    rng = np.random.default_rng(22 + B)
+    A, F, T = 5, 2, 128
+    x = O.ci8_gaussian((A, F, T, 2), rng=rng)
+    w0 = crand(rng, B, A, F, 1, 2)
+    g0 = crand(rng, A, F, 2)
+    g = dev(g0).requires_grad_(True)
+    SteeredStokesIPower.apply(dev(x), dev(w0), g, lam, precision).backward()
+    got = host(g.grad)
+    wd, gd = w0.astype(np.complex128), g0.astype(np.complex128)
+
+    def f(gv):
+        u = gv / np.abs(gv)
+        return O.stokes_i_loss_and_grad_phasors(x, wd * u[None, :, :, None, :], lam)[0]
+
+    h = 1e-6
+    rms = np.sqrt(np.mean(np.abs(got) ** 2))
+    for idx in [(0, 0, 0), (4, 1, 1), (2, 0, 1), (3, 1, 0)]:
+        e = np.zeros_like(gd)
+        e[idx] = h
+        dre = (f(gd + e) - f(gd - e)) / (2 * h)
+        dim = (f(gd + 1j * e) - f(gd - 1j * e)) / (2 * h)
+        assert abs(got[idx].real - dre) / rms < 1e-4
+        assert abs(got[idx].imag - dim) / rms < 1e-4
+
+
+@pytest.mark.parametrize("B,precision", [(4, "f32"), (4, "f16x2"), (9, "f32")])
+def test_steered_power_matches_torch_chain_rule(B, precision):
+    # The fused gain path against torch autograd through steer_phasors and
+    # StokesIPower, on the same kernels: same loss, same gain gradient.
+    import torch
+    from blade_amd.calibration import SteeredStokesIPower, steer_phasors, stokes_i_power
+    rng = np.random.default_rng(30 + B)
+    A, F, T = 42, 3, 512
+    x = dev(O.ci8_gaussian((A, F, T, 2), rng=rng))
+    w = dev(crand(rng, B, A, F, 1, 2))
+    g0 = crand(rng, A, F, 2)
+    lam = [1.0] + [0.3] * (B - 1)
+    g1 = dev(g0).requires_grad_(True)
+    l1 = SteeredStokesIPower.apply(x, w, g1, lam, precision)
+    l1.backward()
+    g2 = dev(g0).requires_grad_(True)
+    l2 = stokes_i_power(x, steer_phasors(w, g2, None, None), lam, precision)
+    l2.backward()
+    assert abs(float(host(l1)) - float(host(l2))) <= 1e-6 * abs(float(host(l2)))
+    assert O.norm_rel_err(host(g1.grad), host(g2.grad)) < 1e-5
+
+
 def calibrator(A, F, T, seed, tau_max=100e-9, snr_db=0.0):
     """Point source plane wave x per-antenna gains + complex noise, quantised to CI8."""
     rng = np.random.default_rng(seed)
