@@ -117,10 +117,60 @@ def steer_phasors(w: torch.Tensor, gains: torch.Tensor | None, delays: torch.Ten
     return out
 
 
+def _replay(step, steps: int, losses: torch.Tensor, graph: bool, device) -> None:
+    """Run ``step`` ``steps`` times into ``losses``: eagerly, or a few eager
+    steps on a side stream and then one captured step replayed."""
+    if not graph:
+        for i in range(steps):
+            losses[i].copy_(step())
+        return
+    side = torch.cuda.Stream(device=device)
+    side.wait_stream(torch.cuda.current_stream(device))
+    warm = min(3, steps)
+    with torch.cuda.stream(side):
+        for i in range(warm):
+            losses[i].copy_(step())
+    torch.cuda.current_stream(device).wait_stream(side)
+    if steps > warm:
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=side):
+            static_loss = step()
+        for i in range(warm, steps):
+            g.replay()
+            losses[i].copy_(static_loss)
+
+
+def _calibrate_gains(x, w, steps, lr, bw, graph, precision):
+    """Gains only, every step on the library's kernels: steering, fused loss +
+    phasor gradient, and the gain gradient with its Adam update in one kernel
+    (``blade_gain_adam``; torch.optim.Adam's arithmetic, betas (0.9, 0.999),
+    eps 1e-8)."""
+    A, F, P = w.shape[1], w.shape[2], w.shape[4]
+    w = w.contiguous()
+    gains = torch.ones((A, F, P), dtype=torch.complex64, device=w.device)
+    exp_avg = torch.zeros_like(gains)
+    exp_avg_sq = torch.zeros_like(gains)
+    count = torch.zeros((), dtype=torch.float32, device=w.device)
+    losses = torch.zeros(steps, dtype=torch.float64, device=w.device)
+    if steps == 0:
+        return gains, None, []
+    ref, _ = ops.stokes_i_loss_and_grad(x, ops.steer_gains(w, gains), bw, precision=precision)
+    ref = abs(float(ref))
+    inv = 1.0 / ref if ref > 0 else 1.0
+
+    def step():
+        loss, grad_w = ops.stokes_i_loss_and_grad(x, ops.steer_gains(w, gains), bw, precision=precision)
+        ops.gain_adam(w, grad_w, gains, exp_avg, exp_avg_sq, count, lr, grad_scale=inv)
+        return loss * inv
+
+    _replay(step, steps, losses, graph, w.device)
+    return gains, None, losses.tolist()
+
+
 def calibrate(x: torch.Tensor, w: torch.Tensor, steps: int = 200, lr: float = 0.05,
               beam_weights=None, fit_gains: bool = True, fit_delays: bool = False,
               freqs_hz: torch.Tensor | None = None, delay_lr: float = 1e-10, graph: bool = True,
-              precision: str = "f32"):
+              precision: str = "f32", fused_adam: bool = True):
     """Adam on unit-modulus gains [A,F,P] (and delays [A]) maximising beam power.
 
     Returns ``(gains, delays, losses)``; the loss is normalised by its value at
@@ -129,7 +179,13 @@ def calibrate(x: torch.Tensor, w: torch.Tensor, steps: int = 200, lr: float = 0.
     (steering, fused loss and gradient, chain rule, Adam) is captured once in
     a CUDA graph after a few eager steps on the same stream and replayed.
     ``precision`` selects the loss + gradient kernel ("f32" or "f16x2").
+    With gains only and ``fused_adam`` the Adam update is the library's
+    ``blade_gain_adam`` kernel; otherwise torch.optim.Adam (capturable) after
+    autograd.
     """
+    if fit_gains and not fit_delays and fused_adam:
+        bw = list(beam_weights) if beam_weights is not None else [1.0] + [0.0] * (w.shape[0] - 1)
+        return _calibrate_gains(x, w, steps, lr, bw, graph, precision)
     A, F, P = w.shape[1], w.shape[2], w.shape[4]
     params = []
     gains = delays = None
@@ -154,6 +210,7 @@ def calibrate(x: torch.Tensor, w: torch.Tensor, steps: int = 200, lr: float = 0.
     bw = list(beam_weights) if beam_weights is not None else [1.0] + [0.0] * (w.shape[0] - 1)
 
     def step():
+        opt.zero_grad(set_to_none=True)
         if gains is not None and delays is None:
             # gains only: steering and its chain rule on the device kernels
             loss = SteeredStokesIPower.apply(x, w, gains, bw, precision)
@@ -163,25 +220,5 @@ def calibrate(x: torch.Tensor, w: torch.Tensor, steps: int = 200, lr: float = 0.
         opt.step()
         return loss.detach() * inv
 
-    if not graph:
-        for i in range(steps):
-            opt.zero_grad(set_to_none=True)
-            losses[i].copy_(step())
-        return gains, delays, losses.tolist()
-    side = torch.cuda.Stream(device=w.device)
-    side.wait_stream(torch.cuda.current_stream(w.device))
-    warm = min(3, steps)
-    with torch.cuda.stream(side):
-        for i in range(warm):
-            opt.zero_grad(set_to_none=True)
-            losses[i].copy_(step())
-    torch.cuda.current_stream(w.device).wait_stream(side)
-    if steps > warm:
-        g = torch.cuda.CUDAGraph()
-        opt.zero_grad(set_to_none=True)
-        with torch.cuda.graph(g, stream=side):
-            static_loss = step()
-        for i in range(warm, steps):
-            g.replay()
-            losses[i].copy_(static_loss)
+    _replay(step, steps, losses, graph, w.device)
     return gains, delays, losses.tolist()

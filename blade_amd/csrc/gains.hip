@@ -12,7 +12,15 @@
 //             tangent to the unit circle.  |g| is floored at 1e-30 as in
 //             g / |g|.clamp_min(1e-30); below the floor u = g * 1e30 and
 //             dL/dconj(g) = G * 1e30.
-// Both are HBM streams over B*A*F*P complex values (one thread per output).
+//   Adam: the gradient above, times 2 (PyTorch's convention for complex
+//             parameters) and a loss scale, then one Adam step on the real
+//             and imaginary parts of g, as torch.optim.Adam (no weight
+//             decay, no amsgrad) computes it:
+//               m = m + (1 - b1)(grad - m),  v = b2 v + (1 - b2) grad^2,
+//               g = g - lr / (1 - b1^t) * m / (sqrt(v) / sqrt(1 - b2^t) + eps),
+//             with the step count t a device scalar (incremented first, so a
+//             captured graph replays successive steps).
+// All are HBM streams over B*A*F*P complex values (one thread per output).
 #include "common.hh"
 
 namespace bl {
@@ -34,11 +42,9 @@ __global__ __launch_bounds__(256) void steer_gains_kernel(const float2* __restri
     out[i] = cmul(w[i], u);
 }
 
-__global__ __launch_bounds__(256) void gain_gradient_kernel(const float2* __restrict__ w, const float2* __restrict__ gw,
-                                                           const float2* __restrict__ g, float2* __restrict__ gg,
-                                                           uint64_t afp, int B) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= afp) return;
+// dL/dconj(g[i]) from the phasors, their gradient and the gains.
+__device__ __forceinline__ float2 gain_grad_at(const float2* __restrict__ w, const float2* __restrict__ gw, float2 gi,
+                                               uint64_t i, uint64_t afp, int B) {
     // G = sum_b conj(w) gw, beams in order.
     float2 G = make_float2(0.f, 0.f);
     for (int b = 0; b < B; b++) {
@@ -47,18 +53,55 @@ __global__ __launch_bounds__(256) void gain_gradient_kernel(const float2* __rest
         G.y += wv.x * gv.y - wv.y * gv.x;
     }
     float norm;
-    const float2 u = unit_gain(g[i], norm);
-    float2 r;
-    if (norm >= kGainFloor) {
-        // (G - u^2 conj(G)) / (2 |g|)
-        const float2 u2 = cmul(u, u);
-        const float2 t = cmul(u2, make_float2(G.x, -G.y));
-        const float s = 0.5f / norm;
-        r = make_float2((G.x - t.x) * s, (G.y - t.y) * s);
-    } else {
-        r = make_float2(G.x / kGainFloor, G.y / kGainFloor);
-    }
-    gg[i] = r;
+    const float2 u = unit_gain(gi, norm);
+    if (norm < kGainFloor) return make_float2(G.x / kGainFloor, G.y / kGainFloor);
+    // (G - u^2 conj(G)) / (2 |g|)
+    const float2 u2 = cmul(u, u);
+    const float2 t = cmul(u2, make_float2(G.x, -G.y));
+    const float s = 0.5f / norm;
+    return make_float2((G.x - t.x) * s, (G.y - t.y) * s);
+}
+
+__global__ __launch_bounds__(256) void gain_gradient_kernel(const float2* __restrict__ w, const float2* __restrict__ gw,
+                                                           const float2* __restrict__ g, float2* __restrict__ gg,
+                                                           uint64_t afp, int B) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= afp) return;
+    gg[i] = gain_grad_at(w, gw, g[i], i, afp, B);
+}
+
+__global__ void adam_tick_kernel(float* step) {
+    if (threadIdx.x == 0) *step += 1.f;
+}
+
+struct AdamArgs {
+    float lr, beta1, beta2, eps, grad_scale;
+};
+
+__device__ __forceinline__ void adam_update(float& p, float& m, float& v, float grad, const AdamArgs& h, float bc1,
+                                            float bc2_sqrt) {
+    m = m + (1.f - h.beta1) * (grad - m);
+    v = h.beta2 * v + (1.f - h.beta2) * grad * grad;
+    p = p - (h.lr / bc1) * m / (sqrtf(v) / bc2_sqrt + h.eps);
+}
+
+__global__ __launch_bounds__(256) void gain_adam_kernel(const float2* __restrict__ w, const float2* __restrict__ gw,
+                                                       float2* __restrict__ g, float2* __restrict__ m,
+                                                       float2* __restrict__ v, const float* __restrict__ step,
+                                                       AdamArgs h, uint64_t afp, int B) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= afp) return;
+    float2 gi = g[i];
+    const float2 d = gain_grad_at(w, gw, gi, i, afp, B);
+    const float t = *step;
+    const float bc1 = 1.f - powf(h.beta1, t);
+    const float bc2_sqrt = sqrtf(1.f - powf(h.beta2, t));
+    float2 mi = m[i], vi = v[i];
+    adam_update(gi.x, mi.x, vi.x, 2.f * h.grad_scale * d.x, h, bc1, bc2_sqrt);
+    adam_update(gi.y, mi.y, vi.y, 2.f * h.grad_scale * d.y, h, bc1, bc2_sqrt);
+    g[i] = gi;
+    m[i] = mi;
+    v[i] = vi;
 }
 
 static int check_gain_args(const char* who, const void* a, const void* b, const void* c, uint64_t B, uint64_t A,
@@ -99,4 +142,27 @@ extern "C" int blade_gain_gradient(const void* phasors, const void* grad_phasors
                        static_cast<const float2*>(grad_phasors), static_cast<const float2*>(gains),
                        static_cast<float2*>(grad_gains), afp, (int)B);
     return check_launch("gain gradient");
+}
+
+extern "C" int blade_gain_adam(const void* phasors, const void* grad_phasors, void* gains, void* exp_avg,
+                               void* exp_avg_sq, float* step, float lr, float beta1, float beta2, float eps,
+                               float grad_scale, uint64_t B, uint64_t A, uint64_t F, uint64_t P,
+                               blade_stream_t stream) {
+    int rc = check_gain_args("gain Adam", phasors, grad_phasors, gains, B, A, F, P);
+    if (!rc) rc = check_gain_args("gain Adam", exp_avg, exp_avg_sq, gains, B, A, F, P);
+    if (rc) return rc;
+    if (!step) return fail("gain Adam: null step counter");
+    if (!(lr > 0.f) || !(beta1 >= 0.f && beta1 < 1.f) || !(beta2 >= 0.f && beta2 < 1.f) || !(eps >= 0.f))
+        return fail("gain Adam: need lr > 0, 0 <= beta1, beta2 < 1 and eps >= 0");
+    const uint64_t afp = A * F * P;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(adam_tick_kernel, dim3(1), dim3(64), 0, s, step);
+    if (afp > 0) {
+        const AdamArgs h{lr, beta1, beta2, eps, grad_scale};
+        hipLaunchKernelGGL(gain_adam_kernel, dim3((unsigned)ceil_div(afp, 256)), dim3(256), 0, s,
+                           static_cast<const float2*>(phasors), static_cast<const float2*>(grad_phasors),
+                           static_cast<float2*>(gains), static_cast<float2*>(exp_avg),
+                           static_cast<float2*>(exp_avg_sq), static_cast<const float*>(step), h, afp, (int)B);
+    }
+    return check_launch("gain Adam");
 }

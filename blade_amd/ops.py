@@ -234,6 +234,25 @@ def gain_gradient(phasors: torch.Tensor, grad_phasors: torch.Tensor, gains: torc
     return out
 
 
+def gain_adam(phasors: torch.Tensor, grad_phasors: torch.Tensor, gains: torch.Tensor, exp_avg: torch.Tensor,
+              exp_avg_sq: torch.Tensor, step: torch.Tensor, lr: float, betas=(0.9, 0.999), eps: float = 1e-8,
+              grad_scale: float = 1.0, stream=None) -> None:
+    """One Adam step on ``gains`` (in place) from grad_phasors = dL/dconj(phasors * gains/|gains|):
+    the gradient ``2 * grad_scale * dL/dconj(gains)`` (PyTorch's convention), Adam
+    states ``exp_avg``/``exp_avg_sq`` (CF32, gains' shape) and ``step`` (F32 device
+    scalar, incremented first), as torch.optim.Adam without weight decay."""
+    B, A, F, P = _gain_shapes(phasors, gains)
+    for name, t in (("grad_phasors", grad_phasors), ("exp_avg", exp_avg), ("exp_avg_sq", exp_avg_sq)):
+        want = phasors.shape if name == "grad_phasors" else gains.shape
+        if tuple(t.shape) != tuple(want) or t.dtype != torch.complex64 or not t.is_contiguous():
+            raise ValueError(f"{name} must be contiguous CF32 {tuple(want)}")
+    if step.dtype != torch.float32 or step.numel() != 1 or step.device != gains.device:
+        raise ValueError("step must be a one-element float32 tensor on the gains' device")
+    call("blade_gain_adam", _ptr(phasors), _ptr(grad_phasors), _ptr(gains), _ptr(exp_avg), _ptr(exp_avg_sq),
+         _ptr(step), float(lr), float(betas[0]), float(betas[1]), float(eps), float(grad_scale), B, A, F, P,
+         stream_handle(stream))
+
+
 def channelizer(x: torch.Tensor, rate: int, out: torch.Tensor | None = None, stream=None):
     """FFT channelizer (Modules::Channelizer), CF32 [A,F,T,P] -> [A,F*C,T/C,P]."""
     A, F, T, P = _shape4(x, "cf32")

@@ -210,6 +210,23 @@ int blade_gain_gradient(const void* phasors, const void* grad_phasors, const voi
                         blade_stream_t stream);
 
 /*
+ * One Adam step on the gains, from grad_phasors as blade_gain_gradient takes it:
+ * grad = 2 * grad_scale * dL/dconj(g) (PyTorch's convention for a complex
+ * parameter, times a loss scale), then, on the real and imaginary parts of g
+ * (CF32 [A,F,P], updated in place) with CF32 exp_avg / exp_avg_sq states of
+ * the same shape and *step a device float incremented first:
+ *   m = m + (1 - beta1)(grad - m), v = beta2 v + (1 - beta2) grad^2,
+ *   g = g - lr / (1 - beta1^t) * m / (sqrt(v) / sqrt(1 - beta2^t) + eps)
+ * (torch.optim.Adam without weight decay or amsgrad).  Allocates nothing;
+ * capturable in a hipGraph (the step count lives on the device).
+ * No reference counterpart (see blade_steer_gains).  For gain calibration callers.
+ */
+int blade_gain_adam(const void* phasors, const void* grad_phasors, void* gains, void* exp_avg,
+                    void* exp_avg_sq, float* step, float lr, float beta1, float beta2, float eps,
+                    float grad_scale, uint64_t B, uint64_t A, uint64_t F, uint64_t P,
+                    blade_stream_t stream);
+
+/*
  * Channelizer — fftshift(FFT_C) of every C-sample chunk along T.
  * Replaces Modules::Channelizer<CF32,CF32>::process (src/modules/channelizer/base.cc:122-146,
  * pre-kernel channelizer.cu:5-19 and cuFFT C2C forward) with the general-rate

@@ -285,3 +285,33 @@ def test_calibration_graph_replays_match_eager_steps():
     _, d_g, l_g = calibrate(xd, wd, steps=12, lr=0.05, fit_delays=True, freqs_hz=freqs, graph=True)
     np.testing.assert_allclose(l_g, l_e, rtol=1e-5, atol=1e-7)
     assert O.norm_rel_err(host(d_g), host(d_e)) <= 1e-5
+
+
+@pytest.mark.parametrize("precision", ["f32", "f16x2"])
+def test_fused_gain_adam_matches_torch_adam(precision):
+    # Gains only: the library's gain Adam kernel against torch.optim.Adam
+    # (capturable) after autograd through SteeredStokesIPower, step by step.
+    from blade_amd.calibration import calibrate
+    A, F, T = 12, 4, 512
+    x, w, _ = calibrator(A, F, T, seed=10)
+    xd, wd = dev(x), dev(w)
+    g1, _, l1 = calibrate(xd, wd, steps=30, lr=0.05, precision=precision, fused_adam=True)
+    g2, _, l2 = calibrate(xd, wd, steps=30, lr=0.05, precision=precision, fused_adam=False)
+    np.testing.assert_allclose(l1, l2, rtol=1e-5, atol=1e-7)
+    assert O.norm_rel_err(host(g1), host(g2)) <= 1e-5
+
+
+def test_gain_adam_rejects_bad_input():
+    import torch
+    from blade_amd import ops
+    from blade_amd._lib import BladeError
+    w = torch.zeros((2, 3, 4, 1, 2), dtype=torch.complex64, device="cuda")
+    g = torch.ones((3, 4, 2), dtype=torch.complex64, device="cuda")
+    m, v = torch.zeros_like(g), torch.zeros_like(g)
+    step = torch.zeros((), dtype=torch.float32, device="cuda")
+    with pytest.raises(BladeError, match="beta"):
+        ops.gain_adam(w, w, g, m, v, step, 0.05, betas=(1.0, 0.999))
+    with pytest.raises(ValueError, match="step"):
+        ops.gain_adam(w, w, g, m, v, step.double(), 0.05)
+    with pytest.raises(ValueError, match="exp_avg"):
+        ops.gain_adam(w, w, g, m[:2], v, step, 0.05)
