@@ -151,7 +151,7 @@ extern "C" int blade_gain_adam(const void* phasors, const void* grad_phasors, vo
     int rc = check_gain_args("gain Adam", phasors, grad_phasors, gains, B, A, F, P);
     if (!rc) rc = check_gain_args("gain Adam", exp_avg, exp_avg_sq, gains, B, A, F, P);
     if (rc) return rc;
-    if (!step) return fail("gain Adam: null step counter");
+    if (!step || (uintptr_t)step % 4) return fail("gain Adam: null or misaligned step counter");
     if (!(lr > 0.f) || !(beta1 >= 0.f && beta1 < 1.f) || !(beta2 >= 0.f && beta2 < 1.f) || !(eps >= 0.f))
         return fail("gain Adam: need lr > 0, 0 <= beta1, beta2 < 1 and eps >= 0");
     const uint64_t afp = A * F * P;
