@@ -340,27 +340,31 @@ def calibration_leg(steps=200):
     lam = [1.0, 0.0, 0.0, 0.0]
     _events_ms(lambda: ops.stokes_i_loss_and_grad(x, w, lam), 5)
     g_ms = _events_ms(lambda: ops.stokes_i_loss_and_grad(x, w, lam), 20)
-    def timed(n, precision):
+    freqs = 1.0e9 + torch.arange(F_, dtype=torch.float64, device="cuda") * 0.5e6
+
+    def timed(n, precision, delays=False):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        out = calibrate(x, w, steps=n, lr=0.05, precision=precision)
+        extra = dict(fit_delays=True, freqs_hz=freqs, delay_lr=1e-10) if delays else {}
+        out = calibrate(x, w, steps=n, lr=0.05, precision=precision, **extra)
         torch.cuda.synchronize()
         return time.perf_counter() - t0, out
 
     res = {}
-    for prec in ("f32", "f16x2"):
-        timed(10, prec)                         # first capture, library set-up
-        t_short, _ = timed(20, prec)
-        t_full, (_, _, losses) = timed(steps, prec)
+    for key, prec, delays in (("f32", "f32", False), ("f16x2", "f16x2", False), ("delays", "f32", True)):
+        timed(10, prec, delays)                 # first capture, library set-up
+        t_short, _ = timed(20, prec, delays)
+        t_full, (_, _, losses) = timed(steps, prec, delays)
         # The per-call set-up (reference loss, 3 eager steps, graph capture)
         # is the same for both runs; the difference is what each further
         # step costs.
-        res[prec] = ((t_full - t_short) * 1e3 / (steps - 20), t_full, losses)
+        res[key] = ((t_full - t_short) * 1e3 / (steps - 20), t_full, losses)
     step_ms, t_full, losses = res["f32"]
     bytes_ = A_ * F_ * T_ * 2 * 2
     return {"workload": "A42 F384 T2048 P2 B4, loss = -Stokes I of beam 0", "grad_ms": round(g_ms, 4),
             "grad_input_gbs": round(bytes_ / g_ms / 1e6, 1), "adam_step_ms": round(step_ms, 3),
             "adam_step_ms_f16x2": round(res["f16x2"][0], 3),
+            "adam_step_ms_gains_and_delays": round(res["delays"][0], 3),
             "loss_last_f16x2": round(res["f16x2"][2][-1], 4),
             "calibrate_total_ms": round(t_full * 1e3, 1), "adam_steps": steps,
             "loss_first_last": [round(losses[0], 4), round(losses[-1], 4)]}

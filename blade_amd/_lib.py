@@ -41,10 +41,11 @@ SIGNATURES = {
     "blade_beamform_batched": [_i, _p, _u64, _p, _p, _u64, _p],
     "blade_beamformer_adjoint": [_p, _p, _p, _u64, _u64, _u64, _u64, _u64, _p],
     "blade_beamformer_stokes_i_grad": [_i, _p, _p, _p, _p, _p, _u64, _u64, _u64, _u64, _u64, _p],
-    "blade_steer_gains": [_p, _p, _p, _u64, _u64, _u64, _u64, _p],
+    "blade_steer_gains": [_p, _p, _p, _p, _p, _u64, _u64, _u64, _u64, _p],
     "blade_gain_gradient": [_p, _p, _p, _p, _u64, _u64, _u64, _u64, _p],
     "blade_gain_adam": [_p, _p, _p, _p, _p, _p, ct.c_float, ct.c_float, ct.c_float, ct.c_float, ct.c_float,
                         _u64, _u64, _u64, _u64, _p],
+    "blade_delay_adam": [_p, _p, _p, _p, _p, _p, _p, _d, _d, _d, _d, _d, _u64, _u64, _u64, _u64, _p],
     "blade_channelizer": [_p, _p, _u64, _u64, _u64, _u64, _u64, _p],
     "blade_channelizer_plan": [_u64, _p],
     "blade_channelizer_detector": [_p, _p, _u64, _u64, _u64, _u64, _u64, _u64, _i, _i, _p],

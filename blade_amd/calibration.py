@@ -140,31 +140,48 @@ def _replay(step, steps: int, losses: torch.Tensor, graph: bool, device) -> None
             losses[i].copy_(static_loss)
 
 
-def _calibrate_gains(x, w, steps, lr, bw, graph, precision):
-    """Gains only, every step on the library's kernels: steering, fused loss +
-    phasor gradient, and the gain gradient with its Adam update in one kernel
-    (``blade_gain_adam``; torch.optim.Adam's arithmetic, betas (0.9, 0.999),
-    eps 1e-8)."""
+def _calibrate_fused(x, w, steps, lr, bw, graph, precision, fit_gains, fit_delays, freqs_hz, delay_lr):
+    """Every step on the library's kernels: steering (``blade_steer_gains``),
+    fused loss + phasor gradient, then the gain gradient with its Adam update
+    (``blade_gain_adam``) and the delay gradient with its FP64 Adam update
+    (``blade_delay_adam``) -- torch.optim.Adam's arithmetic, betas
+    (0.9, 0.999), eps 1e-8."""
     A, F, P = w.shape[1], w.shape[2], w.shape[4]
     w = w.contiguous()
-    gains = torch.ones((A, F, P), dtype=torch.complex64, device=w.device)
-    exp_avg = torch.zeros_like(gains)
-    exp_avg_sq = torch.zeros_like(gains)
-    count = torch.zeros((), dtype=torch.float32, device=w.device)
-    losses = torch.zeros(steps, dtype=torch.float64, device=w.device)
+    dev = w.device
+    gains = torch.ones((A, F, P), dtype=torch.complex64, device=dev) if fit_gains else None
+    delays = torch.zeros((A,), dtype=torch.float64, device=dev) if fit_delays else None
+    if fit_gains:
+        g_m, g_v = torch.zeros_like(gains), torch.zeros_like(gains)
+        g_step = torch.zeros((), dtype=torch.float32, device=dev)
+    if fit_delays:
+        freqs_hz = freqs_hz.to(device=dev, dtype=torch.float64).contiguous()
+        d_m, d_v = torch.zeros_like(delays), torch.zeros_like(delays)
+        d_step = torch.zeros((), dtype=torch.float32, device=dev)
+    losses = torch.zeros(steps, dtype=torch.float64, device=dev)
     if steps == 0:
-        return gains, None, []
-    ref, _ = ops.stokes_i_loss_and_grad(x, ops.steer_gains(w, gains), bw, precision=precision)
+        return gains, delays, []
+
+    def steered():
+        # with both, the gain gradient needs the delay-steered phasors w r
+        w_r = ops.steer_gains(w, None, delays, freqs_hz) if fit_delays else w
+        return w_r, (ops.steer_gains(w_r, gains) if fit_gains else w_r)
+
+    ref, _ = ops.stokes_i_loss_and_grad(x, steered()[1], bw, precision=precision)
     ref = abs(float(ref))
     inv = 1.0 / ref if ref > 0 else 1.0
 
     def step():
-        loss, grad_w = ops.stokes_i_loss_and_grad(x, ops.steer_gains(w, gains), bw, precision=precision)
-        ops.gain_adam(w, grad_w, gains, exp_avg, exp_avg_sq, count, lr, grad_scale=inv)
+        w_r, w_eff = steered()
+        loss, grad_w = ops.stokes_i_loss_and_grad(x, w_eff, bw, precision=precision)
+        if fit_gains:
+            ops.gain_adam(w_r, grad_w, gains, g_m, g_v, g_step, lr, grad_scale=inv)
+        if fit_delays:
+            ops.delay_adam(w_eff, grad_w, freqs_hz, delays, d_m, d_v, d_step, delay_lr, grad_scale=inv)
         return loss * inv
 
-    _replay(step, steps, losses, graph, w.device)
-    return gains, None, losses.tolist()
+    _replay(step, steps, losses, graph, dev)
+    return gains, delays, losses.tolist()
 
 
 def calibrate(x: torch.Tensor, w: torch.Tensor, steps: int = 200, lr: float = 0.05,
@@ -179,13 +196,15 @@ def calibrate(x: torch.Tensor, w: torch.Tensor, steps: int = 200, lr: float = 0.
     (steering, fused loss and gradient, chain rule, Adam) is captured once in
     a CUDA graph after a few eager steps on the same stream and replayed.
     ``precision`` selects the loss + gradient kernel ("f32" or "f16x2").
-    With gains only and ``fused_adam`` the Adam update is the library's
-    ``blade_gain_adam`` kernel; otherwise torch.optim.Adam (capturable) after
-    autograd.
+    With ``fused_adam`` every step is the library's kernels, the Adam updates
+    included (``blade_gain_adam``, ``blade_delay_adam``); otherwise
+    torch.optim.Adam (capturable) after autograd.
     """
-    if fit_gains and not fit_delays and fused_adam:
+    if fit_delays and freqs_hz is None:
+        raise ValueError("delay calibration needs the channel frequencies")
+    if fused_adam and (fit_gains or fit_delays):
         bw = list(beam_weights) if beam_weights is not None else [1.0] + [0.0] * (w.shape[0] - 1)
-        return _calibrate_gains(x, w, steps, lr, bw, graph, precision)
+        return _calibrate_fused(x, w, steps, lr, bw, graph, precision, fit_gains, fit_delays, freqs_hz, delay_lr)
     A, F, P = w.shape[1], w.shape[2], w.shape[4]
     params = []
     gains = delays = None

@@ -4,7 +4,10 @@
 // coefficients (src/modules/phasor/ata.cc:188-204, the `cal` factor); these
 // kernels let that factor be learned as unit-modulus gains u = g / |g|.
 //
-//   steer:    w_eff[b,a,f,0,p] = w[b,a,f,0,p] * u[a,f,p]
+//   steer:    w_eff[b,a,f,0,p] = w[b,a,f,0,p] * u[a,f,p] * r[a,f], with
+//             r = exp(-j 2 pi freq[f] tau[a]) for per-antenna delays tau (FP64
+//             phase, reduced to turns before sincospi as the phasor kernel
+//             does); either factor may be absent.
 //   gradient: given gw = dL/dconj(w_eff) for a real loss L,
 //             G = dL/dconj(u) = sum_b conj(w) gw, and through u = g / |g|
 //             (dconj(u)/dconj(g) = 1 / (2|g|), du/dconj(g) = -u^2 / (2|g|))
@@ -20,6 +23,12 @@
 //               g = g - lr / (1 - b1^t) * m / (sqrt(v) / sqrt(1 - b2^t) + eps),
 //             with the step count t a device scalar (incremented first, so a
 //             captured graph replays successive steps).
+//   delay Adam: with S[a,f,p] = sum_b conj(w_eff) gw, and w_eff depending
+//             on tau through r, dL/dtau_a = 2 Re sum gw conj(dw_eff/dtau)
+//             = -4 pi sum_{f,p} freq[f] Im S[a,f,p]; times a loss scale, then
+//             one Adam step on tau in FP64 (torch.optim.Adam on an FP64
+//             parameter), one workgroup per antenna (FP64 sums in a fixed
+//             order).
 // All are HBM streams over B*A*F*P complex values (one thread per output).
 #include "common.hh"
 
@@ -33,13 +42,32 @@ __device__ __forceinline__ float2 unit_gain(float2 g, float& n) {
     return make_float2(g.x * d, g.y * d);
 }
 
+// exp(-j 2 pi freq tau), the phase in FP64 turns reduced to [-1/2, 1/2].
+__device__ __forceinline__ float2 delay_rot(double freq, double tau) {
+    const double turns = -freq * tau;
+    const double r = turns - rint(turns);
+    double sn, cs;
+    sincospi(2.0 * r, &sn, &cs);
+    return make_float2((float)cs, (float)sn);
+}
+
 __global__ __launch_bounds__(256) void steer_gains_kernel(const float2* __restrict__ w, const float2* __restrict__ g,
-                                                         float2* __restrict__ out, uint64_t n, uint64_t afp) {
+                                                         const double* __restrict__ tau,
+                                                         const double* __restrict__ freq, float2* __restrict__ out,
+                                                         uint64_t n, uint64_t afp, int F, int P) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float norm;
-    const float2 u = unit_gain(g[i % afp], norm);
-    out[i] = cmul(w[i], u);
+    const uint64_t e = i % afp;                       // (a, f, p)
+    float2 v = w[i];
+    if (g) {
+        float norm;
+        v = cmul(v, unit_gain(g[e], norm));
+    }
+    if (tau) {
+        const uint64_t af = e / P;
+        v = cmul(v, delay_rot(freq[af % F], tau[af / F]));
+    }
+    out[i] = v;
 }
 
 // dL/dconj(g[i]) from the phasors, their gradient and the gains.
@@ -104,6 +132,45 @@ __global__ __launch_bounds__(256) void gain_adam_kernel(const float2* __restrict
     v[i] = vi;
 }
 
+__global__ __launch_bounds__(256) void delay_adam_kernel(const float2* __restrict__ weff,
+                                                        const float2* __restrict__ gw,
+                                                        const double* __restrict__ freq, double* __restrict__ tau,
+                                                        double* __restrict__ m, double* __restrict__ v,
+                                                        const float* __restrict__ step, double lr, double beta1,
+                                                        double beta2, double eps, double grad_scale, int B, int F,
+                                                        int P, uint64_t afp) {
+    __shared__ double s_sum[256];
+    const int a = blockIdx.x;
+    double acc = 0.0;
+    // -4 pi sum_{f,p} freq Im(sum_b conj(w_eff) gw): fp over the threads, beams in order.
+    for (int fp = threadIdx.x; fp < F * P; fp += blockDim.x) {
+        const uint64_t e = (uint64_t)a * F * P + fp;
+        double im = 0.0;
+        for (int b = 0; b < B; b++) {
+            const float2 wv = weff[b * afp + e], gv = gw[b * afp + e];
+            im += (double)wv.x * gv.y - (double)wv.y * gv.x;
+        }
+        acc += freq[fp / P] * im;
+    }
+    s_sum[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) s_sum[threadIdx.x] += s_sum[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double grad = grad_scale * (-4.0 * 3.14159265358979323846 * s_sum[0]);
+        const double t = (double)*step;
+        const double bc1 = 1.0 - pow(beta1, t), bc2_sqrt = sqrt(1.0 - pow(beta2, t));
+        double mi = m[a], vi = v[a];
+        mi = mi + (1.0 - beta1) * (grad - mi);
+        vi = beta2 * vi + (1.0 - beta2) * grad * grad;
+        m[a] = mi;
+        v[a] = vi;
+        tau[a] = tau[a] - (lr / bc1) * mi / (sqrt(vi) / bc2_sqrt + eps);
+    }
+}
+
 static int check_gain_args(const char* who, const void* a, const void* b, const void* c, uint64_t B, uint64_t A,
                            uint64_t F, uint64_t P) {
     if (!a || !b || !c) return fail(std::string(who) + ": null pointer");
@@ -117,15 +184,20 @@ static int check_gain_args(const char* who, const void* a, const void* b, const 
 
 using namespace bl;
 
-extern "C" int blade_steer_gains(const void* phasors, const void* gains, void* output, uint64_t B, uint64_t A,
-                                 uint64_t F, uint64_t P, blade_stream_t stream) {
-    int rc = check_gain_args("gain steering", phasors, gains, output, B, A, F, P);
+extern "C" int blade_steer_gains(const void* phasors, const void* gains, const double* delays,
+                                 const double* freqs_hz, void* output, uint64_t B, uint64_t A, uint64_t F,
+                                 uint64_t P, blade_stream_t stream) {
+    int rc = check_gain_args("gain steering", phasors, output, output, B, A, F, P);
     if (rc) return rc;
+    if (gains && (uintptr_t)gains % 8) return fail("gain steering: misaligned gains");
+    if (delays && (!freqs_hz || (uintptr_t)delays % 8 || (uintptr_t)freqs_hz % 8))
+        return fail("gain steering: delays need aligned FP64 delays and channel frequencies");
     const uint64_t afp = A * F * P, n = B * afp;
     if (n == 0) return BLADE_SUCCESS;
     hipLaunchKernelGGL(steer_gains_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0,
                        reinterpret_cast<hipStream_t>(stream), static_cast<const float2*>(phasors),
-                       static_cast<const float2*>(gains), static_cast<float2*>(output), n, afp);
+                       static_cast<const float2*>(gains), delays, delays ? freqs_hz : nullptr,
+                       static_cast<float2*>(output), n, afp, (int)F, (int)P);
     return check_launch("gain steering");
 }
 
@@ -165,4 +237,24 @@ extern "C" int blade_gain_adam(const void* phasors, const void* grad_phasors, vo
                            static_cast<float2*>(exp_avg_sq), static_cast<const float*>(step), h, afp, (int)B);
     }
     return check_launch("gain Adam");
+}
+
+extern "C" int blade_delay_adam(const void* steered, const void* grad_phasors, const double* freqs_hz,
+                                double* delays, double* exp_avg, double* exp_avg_sq, float* step, double lr,
+                                double beta1, double beta2, double eps, double grad_scale, uint64_t B,
+                                uint64_t A, uint64_t F, uint64_t P, blade_stream_t stream) {
+    int rc = check_gain_args("delay Adam", steered, grad_phasors, freqs_hz, B, A, F, P);
+    if (!rc) rc = check_gain_args("delay Adam", delays, exp_avg, exp_avg_sq, B, A, F, P);
+    if (rc) return rc;
+    if (!step || (uintptr_t)step % 4) return fail("delay Adam: null or misaligned step counter");
+    if (!(lr > 0.0) || !(beta1 >= 0.0 && beta1 < 1.0) || !(beta2 >= 0.0 && beta2 < 1.0) || !(eps >= 0.0))
+        return fail("delay Adam: need lr > 0, 0 <= beta1, beta2 < 1 and eps >= 0");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(adam_tick_kernel, dim3(1), dim3(64), 0, s, step);
+    if (A > 0)
+        hipLaunchKernelGGL(delay_adam_kernel, dim3((unsigned)A), dim3(256), 0, s,
+                           static_cast<const float2*>(steered), static_cast<const float2*>(grad_phasors), freqs_hz,
+                           delays, exp_avg, exp_avg_sq, static_cast<const float*>(step), lr, beta1, beta2, eps,
+                           grad_scale, (int)B, (int)F, (int)P, A * F * P);
+    return check_launch("delay Adam");
 }

@@ -213,11 +213,31 @@ def _gain_shapes(phasors: torch.Tensor, gains: torch.Tensor):
     return B, A, F, P
 
 
-def steer_gains(phasors: torch.Tensor, gains: torch.Tensor, stream=None) -> torch.Tensor:
-    """phasors * gains / |gains| (unit-modulus gains [A,F,P] on phasors [B,A,F,1,P])."""
-    B, A, F, P = _gain_shapes(phasors, gains)
+def _delay_shapes(A: int, F: int, delays: torch.Tensor, freqs_hz: torch.Tensor | None):
+    if tuple(delays.shape) != (A,) or delays.dtype != torch.float64 or not delays.is_contiguous():
+        raise ValueError(f"delays must be contiguous float64 [{A}], got {tuple(delays.shape)} {delays.dtype}")
+    if freqs_hz is None or tuple(freqs_hz.shape) != (F,) or freqs_hz.dtype != torch.float64 \
+            or not freqs_hz.is_contiguous():
+        raise ValueError(f"delays need contiguous float64 channel frequencies [{F}]")
+
+
+def steer_gains(phasors: torch.Tensor, gains: torch.Tensor | None, delays: torch.Tensor | None = None,
+                freqs_hz: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+    """phasors * gains/|gains| * exp(-j 2 pi freqs_hz[f] delays[a]): unit-modulus
+    gains [A,F,P] and/or FP64 delays [A] (seconds) on phasors [B,A,F,1,P]."""
+    if gains is not None:
+        B, A, F, P = _gain_shapes(phasors, gains)
+    else:
+        if phasors.dim() != 5 or phasors.shape[3] != 1 or phasors.dtype != torch.complex64 \
+                or not phasors.is_contiguous():
+            raise ValueError("phasors must be contiguous CF32 [B,A,F,1,P]")
+        B, A, F, _, P = phasors.shape
+    if delays is not None:
+        _delay_shapes(A, F, delays, freqs_hz)
     out = torch.empty_like(phasors)
-    call("blade_steer_gains", _ptr(phasors), _ptr(gains), _ptr(out), B, A, F, P, stream_handle(stream))
+    call("blade_steer_gains", _ptr(phasors), _ptr(gains) if gains is not None else None,
+         _ptr(delays) if delays is not None else None, _ptr(freqs_hz) if delays is not None else None,
+         _ptr(out), B, A, F, P, stream_handle(stream))
     return out
 
 
@@ -251,6 +271,29 @@ def gain_adam(phasors: torch.Tensor, grad_phasors: torch.Tensor, gains: torch.Te
     call("blade_gain_adam", _ptr(phasors), _ptr(grad_phasors), _ptr(gains), _ptr(exp_avg), _ptr(exp_avg_sq),
          _ptr(step), float(lr), float(betas[0]), float(betas[1]), float(eps), float(grad_scale), B, A, F, P,
          stream_handle(stream))
+
+
+def delay_adam(steered: torch.Tensor, grad_phasors: torch.Tensor, freqs_hz: torch.Tensor, delays: torch.Tensor,
+               exp_avg: torch.Tensor, exp_avg_sq: torch.Tensor, step: torch.Tensor, lr: float,
+               betas=(0.9, 0.999), eps: float = 1e-8, grad_scale: float = 1.0, stream=None) -> None:
+    """One FP64 Adam step on ``delays`` [A] (in place) from steered = steer_gains(...)
+    and grad_phasors = dL/dconj(steered); ``exp_avg``/``exp_avg_sq`` float64 [A],
+    ``step`` F32 device scalar (incremented first), as torch.optim.Adam."""
+    if steered.dim() != 5 or steered.dtype != torch.complex64 or not steered.is_contiguous():
+        raise ValueError("steered phasors must be contiguous CF32 [B,A,F,1,P]")
+    B, A, F, _, P = steered.shape
+    if tuple(grad_phasors.shape) != tuple(steered.shape) or grad_phasors.dtype != torch.complex64 \
+            or not grad_phasors.is_contiguous():
+        raise ValueError("grad_phasors must be contiguous CF32 of the steered phasors' shape")
+    _delay_shapes(A, F, delays, freqs_hz)
+    for name, t in (("exp_avg", exp_avg), ("exp_avg_sq", exp_avg_sq)):
+        if tuple(t.shape) != (A,) or t.dtype != torch.float64 or not t.is_contiguous():
+            raise ValueError(f"{name} must be contiguous float64 [{A}]")
+    if step.dtype != torch.float32 or step.numel() != 1 or step.device != delays.device:
+        raise ValueError("step must be a one-element float32 tensor on the delays' device")
+    call("blade_delay_adam", _ptr(steered), _ptr(grad_phasors), _ptr(freqs_hz), _ptr(delays), _ptr(exp_avg),
+         _ptr(exp_avg_sq), _ptr(step), float(lr), float(betas[0]), float(betas[1]), float(eps),
+         float(grad_scale), B, A, F, P, stream_handle(stream))
 
 
 def channelizer(x: torch.Tensor, rate: int, out: torch.Tensor | None = None, stream=None):

@@ -188,15 +188,19 @@ int blade_beamformer_stokes_i_grad(int precision, const void* input, const void*
                                    blade_stream_t stream);
 
 /*
- * Gain steering for calibration: output[b,a,f,0,p] = phasors[b,a,f,0,p] * u[a,f,p],
+ * Gain and delay steering for calibration:
+ *   output[b,a,f,0,p] = phasors[b,a,f,0,p] * u[a,f,p] * exp(-j 2 pi freqs_hz[f] delays[a]),
  * u = gains / |gains| (|gains| floored at 1e-30).  CF32 phasors and output
- * [B,A,F,1,P], CF32 gains [A,F,P]; P = 2.
+ * [B,A,F,1,P], CF32 gains [A,F,P] or null (no gain factor), FP64 delays [A] in
+ * seconds or null (no delay factor; freqs_hz FP64 [F] needed with delays; the
+ * phase is formed in FP64 turns).  P = 2.
  * No reference counterpart: the reference's Phasor applies fixed calibration
  * coefficients (src/modules/phasor/ata.cc:188-204); this makes them unit-modulus
- * gains that calibration can learn.  For gain calibration callers.
+ * gains and delays that calibration can learn.  For gain calibration callers.
  */
-int blade_steer_gains(const void* phasors, const void* gains, void* output, uint64_t B, uint64_t A,
-                      uint64_t F, uint64_t P, blade_stream_t stream);
+int blade_steer_gains(const void* phasors, const void* gains, const double* delays,
+                      const double* freqs_hz, void* output, uint64_t B, uint64_t A, uint64_t F,
+                      uint64_t P, blade_stream_t stream);
 
 /*
  * Gradient through the gain steering: given grad_phasors = dL/dconj(w_eff) of a
@@ -225,6 +229,20 @@ int blade_gain_adam(const void* phasors, const void* grad_phasors, void* gains, 
                     void* exp_avg_sq, float* step, float lr, float beta1, float beta2, float eps,
                     float grad_scale, uint64_t B, uint64_t A, uint64_t F, uint64_t P,
                     blade_stream_t stream);
+
+/*
+ * One Adam step on per-antenna delays (FP64 [A], seconds, updated in place)
+ * from steered = blade_steer_gains' output and grad_phasors = dL/dconj(steered):
+ *   grad_a = grad_scale * dL/dtau_a = -4 pi grad_scale sum_{b,f,p} freqs_hz[f] Im(conj(steered) grad_phasors)
+ * then the FP64 form of blade_gain_adam's update with FP64 exp_avg /
+ * exp_avg_sq [A] and its own *step (device float, incremented first).
+ * Allocates nothing; capturable.  No reference counterpart (see
+ * blade_steer_gains).  For delay calibration callers.
+ */
+int blade_delay_adam(const void* steered, const void* grad_phasors, const double* freqs_hz,
+                     double* delays, double* exp_avg, double* exp_avg_sq, float* step, double lr,
+                     double beta1, double beta2, double eps, double grad_scale, uint64_t B,
+                     uint64_t A, uint64_t F, uint64_t P, blade_stream_t stream);
 
 /*
  * Channelizer — fftshift(FFT_C) of every C-sample chunk along T.

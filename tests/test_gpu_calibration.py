@@ -179,6 +179,16 @@ def test_steer_gains_kernel():
     assert O.norm_rel_err(got, ref) < 1e-6
     with pytest.raises(ValueError, match="gains"):
         ops.steer_gains(dev(w0), dev(g0[:4]))
+    # per-antenna delays, with and without gains: exp(-j 2 pi f tau) from an FP64 phase
+    tau = rng.uniform(-100e-9, 100e-9, size=5)
+    freqs = 6.5e9 + np.arange(4) * 0.5e6
+    rot = np.exp(-2j * np.pi * freqs[None, :] * tau[:, None])                # [A, F]
+    got = host(ops.steer_gains(dev(w0), None, dev(tau), dev(freqs)))
+    assert O.norm_rel_err(got, w0.astype(np.complex128) * rot[None, :, :, None, None]) < 1e-6
+    got = host(ops.steer_gains(dev(w0), dev(g0), dev(tau), dev(freqs)))
+    assert O.norm_rel_err(got, ref * rot[None, :, :, None, None]) < 1e-6
+    with pytest.raises(ValueError, match="frequencies"):
+        ops.steer_gains(dev(w0), None, dev(tau), None)
 
 
 @pytest.mark.parametrize("B,lam,precision", [(2, [1.0, 0.5], "f32"), (2, [1.0, 0.5], "f16x2"),
@@ -285,6 +295,31 @@ def test_calibration_graph_replays_match_eager_steps():
     _, d_g, l_g = calibrate(xd, wd, steps=12, lr=0.05, fit_delays=True, freqs_hz=freqs, graph=True)
     np.testing.assert_allclose(l_g, l_e, rtol=1e-5, atol=1e-7)
     assert O.norm_rel_err(host(d_g), host(d_e)) <= 1e-5
+
+
+@pytest.mark.parametrize("fit_gains,precision", [(False, "f32"), (True, "f32"), (True, "f16x2")])
+def test_fused_delay_adam_matches_torch_adam(fit_gains, precision):
+    # Delays (and gains): the library's steering and Adam kernels against
+    # torch autograd through steer_phasors and torch.optim.Adam, step by step.
+    import torch
+    from blade_amd.calibration import calibrate
+    A, F, T = 12, 4, 512
+    x, w, _ = calibrator(A, F, T, seed=11)
+    xd, wd = dev(x), dev(w)
+    freqs = torch.linspace(1.0e9, 1.0015e9, F, dtype=torch.float64, device="cuda")
+    kw = dict(steps=20, lr=0.05, fit_gains=fit_gains, fit_delays=True, freqs_hz=freqs, delay_lr=2e-10,
+              precision=precision)
+    g1, d1, l1 = calibrate(xd, wd, fused_adam=True, **kw)
+    g2, d2, l2 = calibrate(xd, wd, fused_adam=False, **kw)
+    np.testing.assert_allclose(l1, l2, rtol=1e-5, atol=1e-7)
+    # The two paths steer in a different order (w r u against (w u) r, the
+    # rotation from reduced FP64 turns against torch.polar), so the FP32
+    # phasors differ in the last bit and 20 coupled Adam steps carry that
+    # into the parameters: 1e-4 normwise, against 1e-5 for the losses.
+    assert O.norm_rel_err(host(d1), host(d2)) <= 1e-4
+    assert abs(host(d1)).max() > 1e-10                     # the delays moved
+    if fit_gains:
+        assert O.norm_rel_err(host(g1), host(g2)) <= 1e-4
 
 
 @pytest.mark.parametrize("precision", ["f32", "f16x2"])
