@@ -171,6 +171,10 @@ __global__ __launch_bounds__(256) void delay_adam_kernel(const float2* __restric
     }
 }
 
+// Shape checks first: empty inputs (a zero axis, whose tensors may have null
+// data pointers) succeed without a launch, as every entry point does.
+static bool gain_shape_empty(uint64_t B, uint64_t A, uint64_t F) { return B == 0 || A == 0 || F == 0; }
+
 static int check_gain_args(const char* who, const void* a, const void* b, const void* c, uint64_t B, uint64_t A,
                            uint64_t F, uint64_t P) {
     if (!a || !b || !c) return fail(std::string(who) + ": null pointer");
@@ -187,6 +191,8 @@ using namespace bl;
 extern "C" int blade_steer_gains(const void* phasors, const void* gains, const double* delays,
                                  const double* freqs_hz, void* output, uint64_t B, uint64_t A, uint64_t F,
                                  uint64_t P, blade_stream_t stream) {
+    if (P != 2) return fail("gain steering: number of polarizations must be 2");
+    if (gain_shape_empty(B, A, F)) return BLADE_SUCCESS;
     int rc = check_gain_args("gain steering", phasors, output, output, B, A, F, P);
     if (rc) return rc;
     if (gains && (uintptr_t)gains % 8) return fail("gain steering: misaligned gains");
@@ -204,6 +210,8 @@ extern "C" int blade_steer_gains(const void* phasors, const void* gains, const d
 extern "C" int blade_gain_gradient(const void* phasors, const void* grad_phasors, const void* gains,
                                    void* grad_gains, uint64_t B, uint64_t A, uint64_t F, uint64_t P,
                                    blade_stream_t stream) {
+    if (P != 2) return fail("gain gradient: number of polarizations must be 2");
+    if (gain_shape_empty(B, A, F)) return BLADE_SUCCESS;
     int rc = check_gain_args("gain gradient", phasors, grad_phasors, gains, B, A, F, P);
     if (!rc) rc = check_gain_args("gain gradient", grad_gains, gains, gains, B, A, F, P);
     if (rc) return rc;
@@ -220,6 +228,8 @@ extern "C" int blade_gain_adam(const void* phasors, const void* grad_phasors, vo
                                void* exp_avg_sq, float* step, float lr, float beta1, float beta2, float eps,
                                float grad_scale, uint64_t B, uint64_t A, uint64_t F, uint64_t P,
                                blade_stream_t stream) {
+    if (P != 2) return fail("gain Adam: number of polarizations must be 2");
+    if (gain_shape_empty(B, A, F)) return BLADE_SUCCESS;
     int rc = check_gain_args("gain Adam", phasors, grad_phasors, gains, B, A, F, P);
     if (!rc) rc = check_gain_args("gain Adam", exp_avg, exp_avg_sq, gains, B, A, F, P);
     if (rc) return rc;
@@ -243,6 +253,8 @@ extern "C" int blade_delay_adam(const void* steered, const void* grad_phasors, c
                                 double* delays, double* exp_avg, double* exp_avg_sq, float* step, double lr,
                                 double beta1, double beta2, double eps, double grad_scale, uint64_t B,
                                 uint64_t A, uint64_t F, uint64_t P, blade_stream_t stream) {
+    if (P != 2) return fail("delay Adam: number of polarizations must be 2");
+    if (gain_shape_empty(B, A, F)) return BLADE_SUCCESS;
     int rc = check_gain_args("delay Adam", steered, grad_phasors, freqs_hz, B, A, F, P);
     if (!rc) rc = check_gain_args("delay Adam", delays, exp_avg, exp_avg_sq, B, A, F, P);
     if (rc) return rc;

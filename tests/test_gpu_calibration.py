@@ -350,3 +350,20 @@ def test_gain_adam_rejects_bad_input():
         ops.gain_adam(w, w, g, m, v, step.double(), 0.05)
     with pytest.raises(ValueError, match="exp_avg"):
         ops.gain_adam(w, w, g, m[:2], v, step, 0.05)
+
+
+def test_gain_entries_accept_empty_shapes():
+    # A zero axis (tensors that may have null data pointers) succeeds without a launch.
+    import torch
+    from blade_amd import ops
+    w = torch.zeros((2, 3, 0, 1, 2), dtype=torch.complex64, device="cuda")
+    g = torch.zeros((3, 0, 2), dtype=torch.complex64, device="cuda")
+    assert ops.steer_gains(w, g).shape == w.shape
+    assert ops.gain_gradient(w, w, g).shape == g.shape
+    step = torch.zeros((), dtype=torch.float32, device="cuda")
+    ops.gain_adam(w, w, g, torch.zeros_like(g), torch.zeros_like(g), step, 0.05)
+    d = torch.zeros((3,), dtype=torch.float64, device="cuda")
+    ops.delay_adam(w, w, torch.zeros((0,), dtype=torch.float64, device="cuda"), d, torch.zeros_like(d),
+                   torch.zeros_like(d), step, 1e-10)
+    torch.cuda.synchronize()
+    assert float(step) == 0.0 and float(d.abs().max()) == 0.0
