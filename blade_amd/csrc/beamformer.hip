@@ -46,6 +46,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 namespace bl {
 
@@ -775,6 +776,42 @@ static int nsteps_for(bool f16x2, int A) {
 
 static int groups_for(int B, int mw) { return (B + mw * (TILE / 2) - 1) / (mw * (TILE / 2)); }
 
+// Chunks per workgroup.  The resident workgroups (occupancy x CUs) run the
+// grid in rounds, and a round takes cpw chunk times, so the estimated time is
+// ceil(workgroups / resident) * cpw chunk times.  The chosen cpw is the
+// largest one within 2% of the best estimate: fewer, longer workgroups pack
+// their phasors fewer times.  Measured at the production shapes: the Mode-B
+// kernels (3 per CU) get one round of 768 workgroups, cpw = 8; the Mode-A
+// kernel (4 per CU) gets three rounds of 1024, cpw = 4 (146 against 150 us
+// with one round of 768).
+static int pick_chunks_per_wg(const void* fn, size_t lds, int n_tchunks, long per_chunk) {
+    static std::mutex mu;
+    static std::vector<std::pair<std::pair<const void*, size_t>, int>> cache;
+    int occ = 0;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (const auto& e : cache)
+            if (e.first.first == fn && e.first.second == lds) occ = e.second;
+        if (!occ) {
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 64 * WAVES, lds) != hipSuccess || occ < 1)
+                occ = 3;
+            cache.push_back({{fn, lds}, occ});
+        }
+    }
+    const long resident = (long)occ * device_cus();
+    const int cmax = std::min(n_tchunks, 4096);
+    auto est = [&](int c) {
+        const long wgs = (long)((n_tchunks + c - 1) / c) * per_chunk;
+        return ((wgs + resident - 1) / resident) * c;
+    };
+    long best_t = est(1);
+    for (int c = 2; c <= cmax; c++) best_t = std::min(best_t, est(c));
+    int best = 1;
+    for (int c = 1; c <= cmax; c++)
+        if (est(c) * 100 <= best_t * 102) best = c;
+    return best;
+}
+
 template <int NT, int MW, bool CI8, bool F16X2, int D, bool NTS, int EPI = 0>
 static int launch_bf(const BfArgs& args, unsigned char* ws, size_t ws_bytes, hipStream_t s) {
     BfArgs a = args;
@@ -799,17 +836,6 @@ static int launch_bf(const BfArgs& args, unsigned char* ws, size_t ws_bytes, hip
     }
     a.sink = reinterpret_cast<float4*>(sink_mem);
     a.n_tchunks = (a.T + span - 1) / span;
-    // Chunks per workgroup: about 768 workgroups (one round at 3 per CU);
-    // measured best for both variants at the production shape.
-    int cpw = env_int("BLADE_BF_CHUNKS_PER_WG", 0);
-    if (cpw <= 0) {
-        const long target = 256L * 3;
-        const long per_chunk = (long)a.F * groups;
-        cpw = (int)std::max(1L, ((long)a.n_tchunks * per_chunk + target - 1) / target);
-        cpw = std::min(cpw, a.n_tchunks);
-    }
-    a.chunks_per_wg = cpw;
-    dim3 grid((unsigned)((a.n_tchunks + cpw - 1) / cpw), (unsigned)a.F, (unsigned)groups);
     const void* fn = reinterpret_cast<const void*>(&beamform_kernel<NT, MW, CI8, F16X2, D, NTS, EPI>);
     if (lds > 150 * 1024) return fail("beamformer: too many antennas for the LDS phasor stage");
     if (lds > 65536) {
@@ -819,6 +845,11 @@ static int launch_bf(const BfArgs& args, unsigned char* ws, size_t ws_bytes, hip
             return BLADE_HIP_ERROR;
         }
     }
+    int cpw = env_int("BLADE_BF_CHUNKS_PER_WG", 0);
+    if (cpw <= 0) cpw = pick_chunks_per_wg(fn, lds, a.n_tchunks, (long)a.F * groups);
+    cpw = std::min(cpw, a.n_tchunks);
+    a.chunks_per_wg = cpw;
+    dim3 grid((unsigned)((a.n_tchunks + cpw - 1) / cpw), (unsigned)a.F, (unsigned)groups);
     hipLaunchKernelGGL((beamform_kernel<NT, MW, CI8, F16X2, D, NTS, EPI>), grid, dim3(64 * WAVES), lds, s, a);
     return check_launch("beamformer");
 }
