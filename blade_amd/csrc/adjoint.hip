@@ -915,7 +915,7 @@ static int run_adjoint(bool fused, bool f16, const void* input, const void* phas
     const long per_split = (long)F * a.btiles;
     long tiles_per;
     if (fused) {
-        const long slots = 256L * (b4 ? SG4_WGS : f16 ? (a.n_at <= 3 ? 3 : 2) : (a.n_at <= 4 ? 4 : 2));
+        const long slots = (long)device_cus() * (b4 ? SG4_WGS : f16 ? (a.n_at <= 3 ? 3 : 2) : (a.n_at <= 4 ? 4 : 2));
         long best = -1;
         tiles_per = tiles;
         for (long tp = tiles; tp >= 1; tp--) {
