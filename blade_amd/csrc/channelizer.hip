@@ -1057,24 +1057,32 @@ __global__ __launch_bounds__(NT, 2) void fft16_kernel(const float4* __restrict__
                     const float2 t = cmul(wr, v);
                     x = make_float2(u.x + sg * t.x, u.y + sg * t.y);
                 };
+                // W_32^(r + 8) = -j W_32^r: one twiddle product serves
+                // results r and r + 8.
+                auto negj = [](float2 w) { return make_float2(w.y, -w.x); };
                 if constexpr (MODE == 1) {
                     // Stokes I: one polarization at a time (half the registers).
 #pragma unroll
                     for (int q = 0; q < 2; q++) {
                         fft16_mid_regs1<C, NSR>(q == 0 ? p0 : p1, jr, twr, a);
 #pragma unroll
-                        for (int r = 0; r < 16; r++) {
-                            radix2(a[r], cmul(tsw, w32(r)));
+                        for (int r = 0; r < 8; r++) {
+                            const float2 wr = cmul(tsw, w32(r));
+                            radix2(a[r], wr);
+                            radix2(a[r + 8], negj(wr));
                             acc[r][0] += a[r].x * a[r].x + a[r].y * a[r].y;
+                            acc[r + 8][0] += a[r + 8].x * a[r + 8].x + a[r + 8].y * a[r + 8].y;
                         }
                     }
                 } else {
                     fft16_mid_regs<C, NSR>(p0, p1, jr, twr, a, b);
 #pragma unroll
-                    for (int r = 0; r < 16; r++) {
-                        const float2 wr = cmul(tsw, w32(r));
+                    for (int r = 0; r < 8; r++) {
+                        const float2 wr = cmul(tsw, w32(r)), wr8 = negj(wr);
                         radix2(a[r], wr);
                         radix2(b[r], wr);
+                        radix2(a[r + 8], wr8);
+                        radix2(b[r + 8], wr8);
                     }
                 }
             } else {
