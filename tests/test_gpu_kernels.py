@@ -112,6 +112,30 @@ def test_beamform_chain(rate, polarize, precision):
     assert O.norm_rel_err(y, ref) <= TOL
 
 
+@pytest.mark.parametrize("precision", ["f32", "f16x2"])
+@pytest.mark.parametrize("rate,polarize", [(4, True), (1, False)])
+def test_beamform_grid_shape_does_not_change_results(monkeypatch, precision, rate, polarize):
+    # Chunks per workgroup (BLADE_BF_CHUNKS_PER_WG, else chosen from the
+    # kernel's occupancy) only regroup the same per-chunk work: every forced
+    # value, dividing the 32 chunks per channel or not, gives the automatic
+    # choice's output bit for bit, and that output matches the oracle.
+    rng = np.random.default_rng(77)
+    A, F, T, B = 28, 4, 8192, 16
+    x8 = O.ci8_gaussian((A, F, T, 2), rng=rng)
+    w = phasors_for(B, A, F, rng).astype(np.complex64)
+    xd, wd = dev(x8), dev(w)
+    monkeypatch.delenv("BLADE_BF_CHUNKS_PER_WG", raising=False)
+    auto = host(ops.beamform_chain(xd, wd, False, False, rate, polarize, precision))
+    ref = O.channelizer(O.beamformer_ata(x8, w, False, False), rate)
+    if polarize:
+        ref = O.polarizer(ref)
+    assert O.norm_rel_err(auto, ref) <= TOL
+    for cpw in (1, 3, 5, 8, 32, 100):
+        monkeypatch.setenv("BLADE_BF_CHUNKS_PER_WG", str(cpw))
+        y = host(ops.beamform_chain(xd, wd, False, False, rate, polarize, precision))
+        assert np.array_equal(y.view(np.uint32), auto.view(np.uint32)), cpw
+
+
 def test_beamform_chain_production_shape():
     # The Mode-B variant the benchmark times (no incoherent beam, rate 4,
     # XY -> LR) at its full shape A28 F192 T8192 B16, both precisions: six
