@@ -46,7 +46,6 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
-#include <vector>
 
 namespace bl {
 
@@ -785,17 +784,21 @@ static int groups_for(int B, int mw) { return (B + mw * (TILE / 2) - 1) / (mw * 
 // kernel (4 per CU) gets three rounds of 1024, cpw = 4 (146 against 150 us
 // with one round of 768).
 static int pick_chunks_per_wg(const void* fn, size_t lds, int n_tchunks, long per_chunk) {
+    // Occupancy per (kernel, LDS bytes), cached in a fixed table (no heap
+    // allocation on the launch path; past 64 entries it is recomputed).
+    struct Occ { const void* fn; size_t lds; int occ; };
     static std::mutex mu;
-    static std::vector<std::pair<std::pair<const void*, size_t>, int>> cache;
+    static Occ cache[64];
+    static int n_cached = 0;
     int occ = 0;
     {
         std::lock_guard<std::mutex> lk(mu);
-        for (const auto& e : cache)
-            if (e.first.first == fn && e.first.second == lds) occ = e.second;
+        for (int k = 0; k < n_cached; k++)
+            if (cache[k].fn == fn && cache[k].lds == lds) occ = cache[k].occ;
         if (!occ) {
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 64 * WAVES, lds) != hipSuccess || occ < 1)
                 occ = 3;
-            cache.push_back({{fn, lds}, occ});
+            if (n_cached < 64) cache[n_cached++] = {fn, lds, occ};
         }
     }
     const long resident = (long)occ * device_cus();
