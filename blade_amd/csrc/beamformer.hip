@@ -184,15 +184,30 @@ __device__ __forceinline__ void bf_pack_lds(const BfArgs& a, const int f, const 
     const int A = a.A;
     if (threadIdx.x < 2) s_max[threadIdx.x] = 0u;
     float mx[2] = {0.f, 0.f};
-    for (int e = threadIdx.x; e < 2 * NB * A; e += blockDim.x) {
-        const int p = e / (NB * A), r = e - p * (NB * A);
-        const int bl_ = r / A, an = r - bl_ * A, b = gz * NB + bl_;
-        float2 w = make_float2(0.f, 0.f);
-        if (b < a.B) w = a.w[(((size_t)b * A + an) * a.F + f) * 2 + p];
-        s_w[e] = w;
-        const float v = fmaxf(fabsf(w.x), fabsf(w.y));
-        mx[0] = p == 0 ? fmaxf(mx[0], v) : mx[0];
-        mx[1] = p == 1 ? fmaxf(mx[1], v) : mx[1];
+    if (((uintptr_t)a.w & 15) == 0) {
+        // Both pols of a (beam, antenna) are one 16-byte load: half the
+        // scattered phasor loads (FP32 124.6 -> 123.6 us, split FP16 114.7
+        // -> 113.0 us per Mode-B call).
+        for (int r = threadIdx.x; r < NB * A; r += blockDim.x) {
+            const int bl_ = r / A, an = r - bl_ * A, b = gz * NB + bl_;
+            float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (b < a.B) w = reinterpret_cast<const float4*>(a.w)[((size_t)b * A + an) * a.F + f];
+            s_w[r] = make_float2(w.x, w.y);
+            s_w[NB * A + r] = make_float2(w.z, w.w);
+            mx[0] = fmaxf(mx[0], fmaxf(fabsf(w.x), fabsf(w.y)));
+            mx[1] = fmaxf(mx[1], fmaxf(fabsf(w.z), fabsf(w.w)));
+        }
+    } else {
+        for (int e = threadIdx.x; e < 2 * NB * A; e += blockDim.x) {
+            const int p = e / (NB * A), r = e - p * (NB * A);
+            const int bl_ = r / A, an = r - bl_ * A, b = gz * NB + bl_;
+            float2 w = make_float2(0.f, 0.f);
+            if (b < a.B) w = a.w[(((size_t)b * A + an) * a.F + f) * 2 + p];
+            s_w[e] = w;
+            const float v = fmaxf(fabsf(w.x), fabsf(w.y));
+            mx[0] = p == 0 ? fmaxf(mx[0], v) : mx[0];
+            mx[1] = p == 1 ? fmaxf(mx[1], v) : mx[1];
+        }
     }
     __syncthreads();
     float scale[2] = {1.f, 1.f}, uns[2] = {1.f, 1.f};
